@@ -1,0 +1,149 @@
+/*
+ * rblas_hip.h — C ABI of the MI355X (gfx950) implementation of rblas's
+ * Level-3 hot path.
+ *
+ * Every entry point below replaces one public rblas function (cited as
+ * reference `file:line`), keeps its argument meaning and order, and adds the
+ * two things a C boundary needs:
+ *   - every Rust slice `&[T]` / `&mut [T]` becomes `(ptr, len)`; `len` is the
+ *     slice length in elements, needed to reproduce rblas's bounds checks;
+ *   - every call returns an `int32_t` status instead of panicking.
+ *
+ * Conventions
+ *   - `usize` -> `size_t`; `f32`/`f64` -> `float`/`double`.
+ *   - Enums are `int32_t` in rblas's declaration order (types.rs:9-72):
+ *       Layout    { RowMajor = 0, ColMajor = 1 }
+ *       Transpose { None = 0, Trans = 1, ConjTrans = 2 }
+ *       Uplo      { Upper = 0, Lower = 1 }
+ *       Side      { Left = 0, Right = 1 }
+ *       Diag      { NonUnit = 0, Unit = 1 }
+ *   - `Complex<T>` (complex.rs:17-24, `#[repr(C)] {re, im}`) is
+ *     `rblas_complex32_t` / `rblas_complex64_t`, interleaved.
+ *   - Host-slice entry points (no suffix) are synchronous like rblas: on
+ *     return the output buffer (host memory) is final. They copy the touched
+ *     extent of each operand to HBM, run the HIP kernels, and copy back.
+ *   - `_dev` entry points take device pointers (HBM) and a `hipStream_t`
+ *     passed as `void *` (NULL = the null stream). They only enqueue work on
+ *     that stream; the caller synchronises.
+ *   - Every entry point is thread-safe. Host-slice calls from different
+ *     threads run on different streams and overlap on the device.
+ */
+#ifndef RBLAS_HIP_H
+#define RBLAS_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- enums */
+enum { RBLAS_ROW_MAJOR = 0, RBLAS_COL_MAJOR = 1 };
+enum { RBLAS_NO_TRANS = 0, RBLAS_TRANS = 1, RBLAS_CONJ_TRANS = 2 };
+enum { RBLAS_UPPER = 0, RBLAS_LOWER = 1 };
+enum { RBLAS_LEFT = 0, RBLAS_RIGHT = 1 };
+enum { RBLAS_NON_UNIT = 0, RBLAS_UNIT = 1 };
+
+/* --------------------------------------------------------------- status
+ * 1..3 are rblas's `BlasError` variants (error.rs:15-38); the rest are
+ * conditions rblas signals by panicking or that only exist on a device. */
+enum {
+  RBLAS_OK = 0,
+  RBLAS_INVALID_LEADING_DIM = 1, /* BlasError::InvalidLeadingDim {which, got, min} */
+  RBLAS_BUFFER_TOO_SMALL = 2,    /* BlasError::BufferTooSmall {which, got, need}   */
+  RBLAS_ZERO_STRIDE = 3,         /* BlasError::ZeroStride {which}                  */
+  RBLAS_PANIC = 10,    /* a precondition whose violation panics in rblas          */
+  RBLAS_BAD_ENUM = 11, /* an enum argument outside its declared range             */
+  RBLAS_BAD_ARG = 12,  /* an argument of a device-only entry point is invalid     */
+  RBLAS_HIP_ERROR = 20,   /* the HIP runtime reported an error                   */
+  RBLAS_ALLOC_FAILED = 21 /* device memory allocation failed                      */
+};
+
+/* A `BlasError` (error.rs:15-38) as a C struct. `which` is the operand name
+ * rblas uses ('a', 'b', 'c', 'x', 'y'); `bound` is `min` for
+ * InvalidLeadingDim and `need` for BufferTooSmall. */
+typedef struct rblas_error {
+  int32_t kind;
+  char which;
+  uint64_t got;
+  uint64_t bound;
+} rblas_error_t;
+
+typedef struct rblas_complex32 { float re, im; } rblas_complex32_t;
+typedef struct rblas_complex64 { double re, im; } rblas_complex64_t;
+
+/* Text of the last non-OK status on the calling thread (never NULL). */
+const char *rblas_last_error_message(void);
+
+/* ------------------------------------------------------------ level-3 gemm
+ * C := alpha * op(A) * op(B) + beta * C.
+ * Replaces `level3::gemm` (level3.rs:32-62). op(A) is m x k, op(B) is k x n,
+ * C is m x n; lda/ldb/ldc are leading dimensions of the *stored* matrices.
+ * Returns RBLAS_PANIC exactly where rblas's `gemm` would panic (the NN
+ * asserts at level3.rs:164-166, the index checks of `materialize` at
+ * level3.rs:179-222, the C slice in `apply_beta` at kernel/gemm.rs:327), and
+ * also where rblas would write C out of bounds through a raw pointer. */
+int32_t rblas_sgemm(int32_t layout, int32_t transa, int32_t transb, size_t m,
+                    size_t n, size_t k, float alpha, const float *a,
+                    size_t a_len, size_t lda, const float *b, size_t b_len,
+                    size_t ldb, float beta, float *c, size_t c_len, size_t ldc);
+int32_t rblas_dgemm(int32_t layout, int32_t transa, int32_t transb, size_t m,
+                    size_t n, size_t k, double alpha, const double *a,
+                    size_t a_len, size_t lda, const double *b, size_t b_len,
+                    size_t ldb, double beta, double *c, size_t c_len,
+                    size_t ldc);
+
+/* Checked gemm. Replaces `level3::try_gemm` (level3.rs:68-103): validates a,
+ * then b, then c with `check_matrix` (level3.rs:107-140) and fills `*err`
+ * (may be NULL) with the first failure; on success runs gemm. */
+int32_t rblas_sgemm_try(int32_t layout, int32_t transa, int32_t transb,
+                        size_t m, size_t n, size_t k, float alpha,
+                        const float *a, size_t a_len, size_t lda,
+                        const float *b, size_t b_len, size_t ldb, float beta,
+                        float *c, size_t c_len, size_t ldc,
+                        rblas_error_t *err);
+int32_t rblas_dgemm_try(int32_t layout, int32_t transa, int32_t transb,
+                        size_t m, size_t n, size_t k, double alpha,
+                        const double *a, size_t a_len, size_t lda,
+                        const double *b, size_t b_len, size_t ldb, double beta,
+                        double *c, size_t c_len, size_t ldc,
+                        rblas_error_t *err);
+
+/* Device-pointer gemm (no rblas counterpart: the HBM-resident form of
+ * `level3::gemm`). Same validation as the host form. */
+int32_t rblas_sgemm_dev(int32_t layout, int32_t transa, int32_t transb,
+                        size_t m, size_t n, size_t k, float alpha,
+                        const float *a, size_t a_len, size_t lda,
+                        const float *b, size_t b_len, size_t ldb, float beta,
+                        float *c, size_t c_len, size_t ldc, void *stream);
+int32_t rblas_dgemm_dev(int32_t layout, int32_t transa, int32_t transb,
+                        size_t m, size_t n, size_t k, double alpha,
+                        const double *a, size_t a_len, size_t lda,
+                        const double *b, size_t b_len, size_t ldb,
+                        double beta, double *c, size_t c_len, size_t ldc,
+                        void *stream);
+
+/* ------------------------------------------------- device memory & streams
+ * Plumbing for callers that keep operands resident in HBM (no rblas
+ * counterpart). All return a status. */
+int32_t rblas_malloc(void **ptr, size_t bytes);
+int32_t rblas_free(void *ptr);
+int32_t rblas_memcpy_htod(void *dst, const void *src, size_t bytes, void *stream);
+int32_t rblas_memcpy_dtoh(void *dst, const void *src, size_t bytes, void *stream);
+int32_t rblas_memcpy_dtod(void *dst, const void *src, size_t bytes, void *stream);
+int32_t rblas_stream_create(void **stream);
+int32_t rblas_stream_destroy(void *stream);
+int32_t rblas_stream_synchronize(void *stream);
+int32_t rblas_event_create(void **event);
+int32_t rblas_event_destroy(void *event);
+int32_t rblas_event_record(void *event, void *stream);
+int32_t rblas_event_elapsed_ms(float *ms, void *start, void *stop);
+int32_t rblas_device_synchronize(void);
+/* Number of visible HIP devices (0 when there is none); never fails. */
+int32_t rblas_device_count(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* RBLAS_HIP_H */

@@ -1,0 +1,187 @@
+"""ORACLE — test infrastructure only (tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg). Never imported by the product package.
+
+Python face of oracle/liboracle.so (a C++ restatement of rblas's CPU
+algorithms, see oracle.cpp) plus restatements of rblas's argument checks:
+
+  check_matrix / try_gemm_check   level3.rs:68-140
+  gemm_panics                     level3.rs:143-222 + kernel/gemm.rs:198-205,322-338
+
+All arrays are numpy, 1-D "slices" exactly as rblas takes them.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+ROW_MAJOR, COL_MAJOR = 0, 1
+NO_TRANS, TRANS, CONJ_TRANS = 0, 1, 2
+
+_lib = None
+
+
+def build() -> str:
+    """Compile liboracle.so with oracle/Makefile."""
+    subprocess.run(["make", "-s", "-C", HERE, "-B", "liboracle.so"], check=True)
+    return LIB_PATH
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        _lib = C.CDLL(LIB_PATH)
+        sz, i = C.c_size_t, C.c_int
+        f32p, f64p = C.POINTER(C.c_float), C.POINTER(C.c_double)
+        for name, t, tp in (("oracle_sgemm", C.c_float, f32p), ("oracle_dgemm", C.c_double, f64p)):
+            fn = getattr(_lib, name)
+            fn.argtypes = [i, i, i, sz, sz, sz, t, tp, sz, tp, sz, t, tp, sz, i]
+            fn.restype = None
+        for name, t, tp in (("oracle_ssyrk", C.c_float, f32p), ("oracle_dsyrk", C.c_double, f64p)):
+            fn = getattr(_lib, name)
+            fn.argtypes = [i, i, sz, sz, t, tp, sz, t, tp, sz, i]
+            fn.restype = None
+        for name, t, tp in (("oracle_strsm", C.c_float, f32p), ("oracle_dtrsm", C.c_double, f64p)):
+            fn = getattr(_lib, name)
+            fn.argtypes = [i, i, i, i, sz, sz, t, tp, sz, tp, sz]
+            fn.restype = None
+        for name, t, tp in (("oracle_cgemm", C.c_float, f32p), ("oracle_zgemm", C.c_double, f64p)):
+            fn = getattr(_lib, name)
+            fn.argtypes = [i, i, sz, sz, sz, t, t, tp, sz, tp, sz, t, t, tp, sz, i]
+            fn.restype = None
+        for pre, t, tp in (("s", C.c_float, f32p), ("d", C.c_double, f64p)):
+            fn = getattr(_lib, f"oracle_{pre}dot")
+            fn.argtypes = [sz, tp, sz, tp, sz]
+            fn.restype = t
+            fn = getattr(_lib, f"oracle_{pre}axpy")
+            fn.argtypes = [sz, t, tp, sz, tp, sz]
+            fn.restype = None
+            fn = getattr(_lib, f"oracle_{pre}nrm2")
+            fn.argtypes = [sz, tp, sz]
+            fn.restype = t
+        _lib.oracle_has_avx2.restype = i
+    return _lib
+
+
+def _ptr(x: np.ndarray):
+    ct = C.c_float if x.dtype == np.float32 else C.c_double
+    return x.ctypes.data_as(C.POINTER(ct))
+
+
+def _pre(x: np.ndarray) -> str:
+    if x.dtype == np.float32:
+        return "s"
+    if x.dtype == np.float64:
+        return "d"
+    raise TypeError(f"oracle: unsupported dtype {x.dtype}")
+
+
+# ----------------------------------------------------------------- compute
+def gemm(layout, ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, threads=1):
+    """level3::gemm restated (in place on c). Caller guarantees in-bounds."""
+    p = _pre(c)
+    getattr(lib(), f"oracle_{p}gemm")(layout, ta, tb, m, n, k, alpha, _ptr(a), lda, _ptr(b), ldb,
+                                      beta, _ptr(c), ldc, threads)
+
+
+def syrk(uplo, trans, n, k, alpha, a, lda, beta, c, ldc, threads=1):
+    getattr(lib(), f"oracle_{_pre(c)}syrk")(uplo, trans, n, k, alpha, _ptr(a), lda, beta, _ptr(c),
+                                           ldc, threads)
+
+
+def trsm(side, uplo, trans, diag, m, n, alpha, a, lda, b, ldb):
+    getattr(lib(), f"oracle_{_pre(b)}trsm")(side, uplo, trans, diag, m, n, alpha, _ptr(a), lda,
+                                           _ptr(b), ldb)
+
+
+def cgemm(ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, threads=1):
+    """complex3::gemm (4M) restated. a/b/c are complex64/complex128 arrays."""
+    real = np.float32 if c.dtype == np.complex64 else np.float64
+    fn = lib().oracle_cgemm if real == np.float32 else lib().oracle_zgemm
+    av, bv, cv = (x.view(real) for x in (a, b, c))
+    fn(ta, tb, m, n, k, complex(alpha).real, complex(alpha).imag, _ptr(av), lda, _ptr(bv), ldb,
+       complex(beta).real, complex(beta).imag, _ptr(cv), ldc, threads)
+
+
+def dot(n, x, incx, y, incy):
+    return getattr(lib(), f"oracle_{_pre(x)}dot")(n, _ptr(x), incx, _ptr(y), incy)
+
+
+def axpy(n, alpha, x, incx, y, incy):
+    getattr(lib(), f"oracle_{_pre(y)}axpy")(n, alpha, _ptr(x), incx, _ptr(y), incy)
+
+
+def nrm2(n, x, incx):
+    return getattr(lib(), f"oracle_{_pre(x)}nrm2")(n, _ptr(x), incx)
+
+
+def has_avx2() -> bool:
+    return bool(lib().oracle_has_avx2())
+
+
+# ------------------------------------------------------------------ checks
+_MASK = (1 << 64) - 1  # usize arithmetic wraps (release build)
+
+
+def check_matrix(layout, which, rows, cols, lead, length):
+    """level3.rs:107-140. Returns None or a tuple (kind, which, got, bound)
+    with kind 'InvalidLeadingDim' or 'BufferTooSmall'."""
+    if rows == 0 or cols == 0:
+        return None
+    if layout == ROW_MAJOR:
+        min_lead, need = cols, ((rows - 1) * lead + cols) & _MASK
+    else:
+        min_lead, need = rows, ((cols - 1) * lead + rows) & _MASK
+    if lead < min_lead:
+        return ("InvalidLeadingDim", which, lead, min_lead)
+    if length < need:
+        return ("BufferTooSmall", which, length, need)
+    return None
+
+
+def try_gemm_check(layout, ta, tb, m, n, k, a_len, lda, b_len, ldb, c_len, ldc):
+    """level3.rs:84-98: a, then b, then c."""
+    at, bt = ta != NO_TRANS, tb != NO_TRANS
+    for which, rows, cols, ld, ln in (("a", k if at else m, m if at else k, lda, a_len),
+                                      ("b", n if bt else k, k if bt else n, ldb, b_len),
+                                      ("c", m, n, ldc, c_len)):
+        e = check_matrix(layout, which, rows, cols, ld, ln)
+        if e:
+            return e
+    return None
+
+
+def gemm_panics(layout, ta, tb, m, n, k, a_len, lda, b_len, ldb, c_len, ldc, alpha, beta):
+    """True where rblas's unchecked gemm panics (or writes C out of bounds)."""
+    at, bt = ta != NO_TRANS, tb != NO_TRANS
+    if layout == ROW_MAJOR:  # level3.rs:56-60
+        at, bt, m, n = bt, at, n, m
+        a_len, lda, b_len, ldb = b_len, ldb, a_len, lda
+    if not at and not bt:  # level3.rs:164-166
+        if not (a_len >= (((k - 1) * lda + m) & _MASK) or k == 0):
+            return True
+        if not (b_len >= (((n - 1) * ldb + k) & _MASK) or n == 0):
+            return True
+        if not (c_len >= (((n - 1) * ldc + m) & _MASK) or n == 0):
+            return True
+        return False
+    # materialize (level3.rs:193-220): largest gathered index must be < len
+    if m > 0 and k > 0:
+        hi = (k - 1) + (m - 1) * lda if at else (m - 1) + (k - 1) * lda
+        if hi >= a_len:
+            return True
+    if n > 0 and k > 0:
+        hi = (n - 1) + (k - 1) * ldb if bt else (k - 1) + (n - 1) * ldb
+        if hi >= b_len:
+            return True
+    if m == 0 or n == 0:
+        return False
+    touches_c = beta != 1 or (k != 0 and alpha != 0)
+    return touches_c and (m - 1) + (n - 1) * ldc >= c_len

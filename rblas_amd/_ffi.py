@@ -1,0 +1,83 @@
+"""ctypes binding of librblas_hip.so (declared in include/rblas_hip.h).
+
+Loading fails loudly when the library has not been built: there is no CPU
+fallback anywhere in this package.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "librblas_hip.so")
+
+sz, i32, vp = C.c_size_t, C.c_int32, C.c_void_p
+f32, f64 = C.c_float, C.c_double
+
+
+class RblasError(C.Structure):
+    """`rblas_error_t` — a BlasError (error.rs:15-38) as a C struct."""
+    _fields_ = [("kind", C.c_int32), ("which", C.c_char), ("got", C.c_uint64),
+                ("bound", C.c_uint64)]
+
+
+class Complex32(C.Structure):
+    _fields_ = [("re", C.c_float), ("im", C.c_float)]
+
+
+class Complex64(C.Structure):
+    _fields_ = [("re", C.c_double), ("im", C.c_double)]
+
+
+def _gemm_args(t):
+    return [i32, i32, i32, sz, sz, sz, t, vp, sz, sz, vp, sz, sz, t, vp, sz, sz]
+
+
+# name -> (argtypes, restype). Every symbol here is declared in the header;
+# tests check that the library exports each of them.
+SIGNATURES: dict[str, tuple[list, object]] = {
+    "rblas_last_error_message": ([], C.c_char_p),
+    "rblas_sgemm": (_gemm_args(f32), i32),
+    "rblas_dgemm": (_gemm_args(f64), i32),
+    "rblas_sgemm_try": (_gemm_args(f32) + [C.POINTER(RblasError)], i32),
+    "rblas_dgemm_try": (_gemm_args(f64) + [C.POINTER(RblasError)], i32),
+    "rblas_sgemm_dev": (_gemm_args(f32) + [vp], i32),
+    "rblas_dgemm_dev": (_gemm_args(f64) + [vp], i32),
+    "rblas_malloc": ([C.POINTER(vp), sz], i32),
+    "rblas_free": ([vp], i32),
+    "rblas_memcpy_htod": ([vp, vp, sz, vp], i32),
+    "rblas_memcpy_dtoh": ([vp, vp, sz, vp], i32),
+    "rblas_memcpy_dtod": ([vp, vp, sz, vp], i32),
+    "rblas_stream_create": ([C.POINTER(vp)], i32),
+    "rblas_stream_destroy": ([vp], i32),
+    "rblas_stream_synchronize": ([vp], i32),
+    "rblas_event_create": ([C.POINTER(vp)], i32),
+    "rblas_event_destroy": ([vp], i32),
+    "rblas_event_record": ([vp, vp], i32),
+    "rblas_event_elapsed_ms": ([C.POINTER(C.c_float), vp, vp], i32),
+    "rblas_device_synchronize": ([], i32),
+    "rblas_device_count": ([], i32),
+}
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    """The loaded library; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                f"{LIB_PATH} is missing: build it with `python -m rblas_amd.build` "
+                "(the HIP library is the only implementation; there is no fallback)")
+        lb = C.CDLL(LIB_PATH)
+        for name, (argtypes, restype) in SIGNATURES.items():
+            fn = getattr(lb, name)
+            fn.argtypes = argtypes
+            fn.restype = restype
+        _lib = lb
+    return _lib
+
+
+def last_error() -> str:
+    return lib().rblas_last_error_message().decode("utf-8", "replace")

@@ -1,0 +1,199 @@
+// C-ABI gemm entry points: level3::gemm / level3::try_gemm (level3.rs:32-103)
+// for host slices, plus the HBM-resident `_dev` form.
+#include <climits>
+
+#include "kernels.hpp"
+#include "runtime.hpp"
+
+namespace rblas {
+namespace {
+
+// Column-major form of one gemm call, after rblas's RowMajor swap
+// (level3.rs:56-60: A<->B, m<->n, transposes swapped).
+template <typename T>
+struct ColMajorCall {
+  bool at, bt;
+  size_t m, n, k;
+  T alpha, beta;
+  const T *a;
+  size_t a_len, lda;
+  const T *b;
+  size_t b_len, ldb;
+  T *c;
+  size_t c_len, ldc;
+};
+
+template <typename T>
+Check normalize(int32_t layout, int32_t ta, int32_t tb, size_t m, size_t n, size_t k, T alpha,
+                const T *a, size_t a_len, size_t lda, const T *b, size_t b_len, size_t ldb,
+                T beta, T *c, size_t c_len, size_t ldc, ColMajorCall<T> &out) {
+  Check bad;
+  if (!valid_layout(layout) || !valid_trans(ta) || !valid_trans(tb)) {
+    bad.status = RBLAS_BAD_ENUM;
+    bad.err = {RBLAS_BAD_ENUM, 0, 0, 0};
+    bad.msg = "gemm: enum argument out of range";
+    return bad;
+  }
+  if (layout == RBLAS_COL_MAJOR)
+    out = {is_transposed(ta), is_transposed(tb), m, n, k, alpha, beta, a, a_len, lda,
+           b, b_len, ldb, c, c_len, ldc};
+  else
+    out = {is_transposed(tb), is_transposed(ta), n, m, k, alpha, beta, b, b_len, ldb,
+           a, a_len, lda, c, c_len, ldc};
+  Check pc = gemm_panic_check(out.at, out.bt, out.m, out.n, out.k, out.a_len, out.lda, out.b_len,
+                              out.ldb, out.c_len, out.ldc, out.alpha == T(0), out.beta == T(1));
+  if (!pc.ok()) return pc;
+  // The kernels index tiles with 32-bit m, n, k.
+  if (out.m > INT_MAX || out.n > INT_MAX || out.k > INT_MAX) {
+    bad.status = RBLAS_BAD_ARG;
+    bad.err = {RBLAS_BAD_ARG, 0, 0, 0};
+    bad.msg = "gemm: m, n and k must each be below 2^31 on the device path";
+    return bad;
+  }
+  return Check{};
+}
+
+// Enqueue the device work of a normalized call whose pointers are in HBM.
+// Mirrors gemm_contig's control flow (kernel/gemm.rs:198-205): nothing when
+// m or n is 0; beta-only update when k is 0 or alpha is 0.
+template <typename T>
+hipError_t enqueue(const ColMajorCall<T> &cc, const T *a, size_t lda, const T *b, size_t ldb, T *c,
+                   size_t ldc, hipStream_t st) {
+  if (cc.m == 0 || cc.n == 0) return hipSuccess;
+  if (cc.k == 0 || cc.alpha == T(0)) {
+    if (cc.beta == T(1)) return hipSuccess;
+    return launch_scale<T>((int64_t)cc.m, (int64_t)cc.n, cc.beta, c, (int64_t)ldc, st);
+  }
+  GemmProblem<T> p{cc.at, cc.bt, (int64_t)cc.m, (int64_t)cc.n, (int64_t)cc.k, cc.alpha, cc.beta,
+                   a, (int64_t)lda, b, (int64_t)ldb, c, (int64_t)ldc};
+  return launch_gemm<T>(p, st);
+}
+
+template <typename T>
+int32_t gemm_host(const ColMajorCall<T> &cc) {
+  if (cc.m == 0 || cc.n == 0) return RBLAS_OK;
+  CtxLease lease;
+  if (!lease.get()) return hip_fail(lease.status(), "stream create");
+  Ctx &ctx = *lease.get();
+  const bool need_ab = cc.k != 0 && cc.alpha != T(0);
+  Staged sa, sb, sc;
+  if (need_ab) {
+    sa.plan(cc.at ? cc.k : cc.m, cc.at ? cc.m : cc.k, cc.lda);
+    sb.plan(cc.bt ? cc.n : cc.k, cc.bt ? cc.k : cc.n, cc.ldb);
+  }
+  sc.plan(cc.m, cc.n, cc.ldc);
+  RBLAS_HIP_TRY(ctx.ensure(0, sa.elems * sizeof(T)), "device buffer A");
+  RBLAS_HIP_TRY(ctx.ensure(1, sb.elems * sizeof(T)), "device buffer B");
+  RBLAS_HIP_TRY(ctx.ensure(2, sc.elems * sizeof(T)), "device buffer C");
+  T *da = (T *)ctx.buf[0], *db = (T *)ctx.buf[1], *dc = (T *)ctx.buf[2];
+  hipStream_t st = ctx.stream;
+  if (need_ab) {
+    RBLAS_HIP_TRY(stage_in(sa, cc.a, da, st), "copy A to device");
+    RBLAS_HIP_TRY(stage_in(sb, cc.b, db, st), "copy B to device");
+  }
+  RBLAS_HIP_TRY(stage_in(sc, (const T *)cc.c, dc, st), "copy C to device");
+  RBLAS_HIP_TRY(enqueue(cc, da, sa.dev_ld, db, sb.dev_ld, dc, sc.dev_ld, st), "gemm launch");
+  RBLAS_HIP_TRY(stage_out(sc, dc, cc.c, st), "copy C to host");
+  RBLAS_HIP_TRY(hipStreamSynchronize(st), "gemm");
+  return RBLAS_OK;
+}
+
+template <typename T>
+int32_t gemm_entry(int32_t layout, int32_t ta, int32_t tb, size_t m, size_t n, size_t k, T alpha,
+                   const T *a, size_t a_len, size_t lda, const T *b, size_t b_len, size_t ldb,
+                   T beta, T *c, size_t c_len, size_t ldc) {
+  ColMajorCall<T> cc;
+  Check chk = normalize(layout, ta, tb, m, n, k, alpha, a, a_len, lda, b, b_len, ldb, beta, c,
+                        c_len, ldc, cc);
+  if (!chk.ok()) return fail(chk, nullptr);
+  return gemm_host(cc);
+}
+
+template <typename T>
+int32_t gemm_try_entry(int32_t layout, int32_t ta, int32_t tb, size_t m, size_t n, size_t k,
+                       T alpha, const T *a, size_t a_len, size_t lda, const T *b, size_t b_len,
+                       size_t ldb, T beta, T *c, size_t c_len, size_t ldc, rblas_error_t *err) {
+  if (err) *err = {RBLAS_OK, 0, 0, 0};
+  if (!valid_layout(layout) || !valid_trans(ta) || !valid_trans(tb)) {
+    Check bad;
+    bad.status = RBLAS_BAD_ENUM;
+    bad.err = {RBLAS_BAD_ENUM, 0, 0, 0};
+    bad.msg = "try_gemm: enum argument out of range";
+    return fail(bad, err);
+  }
+  Check chk = try_gemm_check(layout, ta, tb, m, n, k, a_len, lda, b_len, ldb, c_len, ldc);
+  if (!chk.ok()) return fail(chk, err);
+  ColMajorCall<T> cc;
+  chk = normalize(layout, ta, tb, m, n, k, alpha, a, a_len, lda, b, b_len, ldb, beta, c, c_len,
+                  ldc, cc);
+  if (!chk.ok()) return fail(chk, err);
+  return gemm_host(cc);
+}
+
+template <typename T>
+int32_t gemm_dev_entry(int32_t layout, int32_t ta, int32_t tb, size_t m, size_t n, size_t k,
+                       T alpha, const T *a, size_t a_len, size_t lda, const T *b, size_t b_len,
+                       size_t ldb, T beta, T *c, size_t c_len, size_t ldc, void *stream) {
+  ColMajorCall<T> cc;
+  Check chk = normalize(layout, ta, tb, m, n, k, alpha, a, a_len, lda, b, b_len, ldb, beta, c,
+                        c_len, ldc, cc);
+  if (!chk.ok()) return fail(chk, nullptr);
+  RBLAS_HIP_TRY(enqueue(cc, cc.a, cc.lda, cc.b, cc.ldb, cc.c, cc.ldc, (hipStream_t)stream),
+                "gemm launch");
+  return RBLAS_OK;
+}
+
+}  // namespace
+}  // namespace rblas
+
+using namespace rblas;
+
+extern "C" {
+
+int32_t rblas_sgemm(int32_t layout, int32_t transa, int32_t transb, size_t m, size_t n, size_t k,
+                    float alpha, const float *a, size_t a_len, size_t lda, const float *b,
+                    size_t b_len, size_t ldb, float beta, float *c, size_t c_len, size_t ldc) {
+  return gemm_entry<float>(layout, transa, transb, m, n, k, alpha, a, a_len, lda, b, b_len, ldb,
+                           beta, c, c_len, ldc);
+}
+
+int32_t rblas_dgemm(int32_t layout, int32_t transa, int32_t transb, size_t m, size_t n, size_t k,
+                    double alpha, const double *a, size_t a_len, size_t lda, const double *b,
+                    size_t b_len, size_t ldb, double beta, double *c, size_t c_len, size_t ldc) {
+  return gemm_entry<double>(layout, transa, transb, m, n, k, alpha, a, a_len, lda, b, b_len, ldb,
+                            beta, c, c_len, ldc);
+}
+
+int32_t rblas_sgemm_try(int32_t layout, int32_t transa, int32_t transb, size_t m, size_t n,
+                        size_t k, float alpha, const float *a, size_t a_len, size_t lda,
+                        const float *b, size_t b_len, size_t ldb, float beta, float *c,
+                        size_t c_len, size_t ldc, rblas_error_t *err) {
+  return gemm_try_entry<float>(layout, transa, transb, m, n, k, alpha, a, a_len, lda, b, b_len,
+                               ldb, beta, c, c_len, ldc, err);
+}
+
+int32_t rblas_dgemm_try(int32_t layout, int32_t transa, int32_t transb, size_t m, size_t n,
+                        size_t k, double alpha, const double *a, size_t a_len, size_t lda,
+                        const double *b, size_t b_len, size_t ldb, double beta, double *c,
+                        size_t c_len, size_t ldc, rblas_error_t *err) {
+  return gemm_try_entry<double>(layout, transa, transb, m, n, k, alpha, a, a_len, lda, b, b_len,
+                                ldb, beta, c, c_len, ldc, err);
+}
+
+int32_t rblas_sgemm_dev(int32_t layout, int32_t transa, int32_t transb, size_t m, size_t n,
+                        size_t k, float alpha, const float *a, size_t a_len, size_t lda,
+                        const float *b, size_t b_len, size_t ldb, float beta, float *c,
+                        size_t c_len, size_t ldc, void *stream) {
+  return gemm_dev_entry<float>(layout, transa, transb, m, n, k, alpha, a, a_len, lda, b, b_len,
+                               ldb, beta, c, c_len, ldc, stream);
+}
+
+int32_t rblas_dgemm_dev(int32_t layout, int32_t transa, int32_t transb, size_t m, size_t n,
+                        size_t k, double alpha, const double *a, size_t a_len, size_t lda,
+                        const double *b, size_t b_len, size_t ldb, double beta, double *c,
+                        size_t c_len, size_t ldc, void *stream) {
+  return gemm_dev_entry<double>(layout, transa, transb, m, n, k, alpha, a, a_len, lda, b, b_len,
+                                ldb, beta, c, c_len, ldc, stream);
+}
+
+}  // extern "C"

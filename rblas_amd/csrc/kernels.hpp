@@ -1,0 +1,39 @@
+// Internal launch interface between the C-ABI host code (api.cpp) and the
+// HIP kernels. Everything here is column-major: the host folds RowMajor into
+// ColMajor with the A<->B, m<->n swap of rblas's level3.rs:56-60 before it
+// calls any of these.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstddef>
+#include <cstdint>
+
+namespace rblas {
+
+// op(A) is m x k, op(B) is k x n, C is m x n (column-major, leading dims of
+// the stored matrices). ta/tb: operand stored transposed (Trans/ConjTrans).
+template <typename T>
+struct GemmProblem {
+  bool ta, tb;
+  int64_t m, n, k;
+  T alpha, beta;
+  const T *a;
+  int64_t lda;
+  const T *b;
+  int64_t ldb;
+  T *c;
+  int64_t ldc;
+};
+
+// C := alpha * op(A) * op(B) + beta * C on MFMA tiles. Requires m, n > 0,
+// k > 0 and alpha != 0 (the host routes the other cases to gemm_scale).
+// beta == 0 never reads C (rblas stores exact zeros, kernel/gemm.rs:328-331).
+template <typename T>
+hipError_t launch_gemm(const GemmProblem<T> &p, hipStream_t stream);
+
+// C := beta * C over an m x n block (rblas apply_beta, kernel/gemm.rs:322-338):
+// beta == 0 stores exact zeros, beta == 1 is a no-op the host skips.
+template <typename T>
+hipError_t launch_scale(int64_t m, int64_t n, T beta, T *c, int64_t ldc,
+                        hipStream_t stream);
+
+}  // namespace rblas

@@ -1,0 +1,162 @@
+// Device context pool, error reporting and the device-memory/stream plumbing
+// entry points of the C ABI.
+#include "runtime.hpp"
+
+#include <mutex>
+#include <vector>
+
+namespace rblas {
+
+namespace {
+thread_local std::string g_msg = "ok";
+std::mutex g_pool_mu;
+std::vector<Ctx *> g_pool;
+}  // namespace
+
+void set_error(const std::string &msg) { g_msg = msg; }
+
+int32_t fail(const Check &c, rblas_error_t *err) {
+  if (err) *err = c.err;
+  if (c.status != RBLAS_OK) set_error(c.msg);
+  return c.status;
+}
+
+int32_t hip_fail(hipError_t e, const char *what) {
+  set_error(std::string(what) + ": " + hipGetErrorString(e));
+  return e == hipErrorOutOfMemory ? RBLAS_ALLOC_FAILED : RBLAS_HIP_ERROR;
+}
+
+hipError_t Ctx::ensure(int slot, size_t bytes) {
+  if (bytes <= cap[slot]) return hipSuccess;
+  if (buf[slot]) {
+    hipError_t e = hipStreamSynchronize(stream);
+    if (e != hipSuccess) return e;
+    hipFree(buf[slot]);
+    buf[slot] = nullptr;
+    cap[slot] = 0;
+  }
+  size_t want = bytes < 4096 ? 4096 : bytes;
+  hipError_t e = hipMalloc(&buf[slot], want);
+  if (e != hipSuccess) {
+    buf[slot] = nullptr;
+    return e;
+  }
+  cap[slot] = want;
+  return hipSuccess;
+}
+
+CtxLease::CtxLease() {
+  {
+    std::lock_guard<std::mutex> g(g_pool_mu);
+    if (!g_pool.empty()) {
+      ctx_ = g_pool.back();
+      g_pool.pop_back();
+      return;
+    }
+  }
+  ctx_ = new Ctx();
+  status_ = hipStreamCreateWithFlags(&ctx_->stream, hipStreamNonBlocking);
+  if (status_ != hipSuccess) {
+    delete ctx_;
+    ctx_ = nullptr;
+  }
+}
+
+CtxLease::~CtxLease() {
+  if (!ctx_) return;
+  std::lock_guard<std::mutex> g(g_pool_mu);
+  g_pool.push_back(ctx_);
+}
+
+}  // namespace rblas
+
+using namespace rblas;
+
+extern "C" {
+
+const char *rblas_last_error_message(void) { return g_msg.c_str(); }
+
+int32_t rblas_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  return n;
+}
+
+int32_t rblas_malloc(void **ptr, size_t bytes) {
+  if (!ptr) return fail(panic("rblas_malloc: null out-pointer"), nullptr);
+  RBLAS_HIP_TRY(hipMalloc(ptr, bytes == 0 ? 1 : bytes), "hipMalloc");
+  return RBLAS_OK;
+}
+
+int32_t rblas_free(void *ptr) {
+  RBLAS_HIP_TRY(hipFree(ptr), "hipFree");
+  return RBLAS_OK;
+}
+
+int32_t rblas_memcpy_htod(void *dst, const void *src, size_t bytes, void *stream) {
+  RBLAS_HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, (hipStream_t)stream),
+                "hipMemcpyAsync(HtoD)");
+  RBLAS_HIP_TRY(hipStreamSynchronize((hipStream_t)stream), "hipStreamSynchronize");
+  return RBLAS_OK;
+}
+
+int32_t rblas_memcpy_dtoh(void *dst, const void *src, size_t bytes, void *stream) {
+  RBLAS_HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, (hipStream_t)stream),
+                "hipMemcpyAsync(DtoH)");
+  RBLAS_HIP_TRY(hipStreamSynchronize((hipStream_t)stream), "hipStreamSynchronize");
+  return RBLAS_OK;
+}
+
+int32_t rblas_memcpy_dtod(void *dst, const void *src, size_t bytes, void *stream) {
+  RBLAS_HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, (hipStream_t)stream),
+                "hipMemcpyAsync(DtoD)");
+  return RBLAS_OK;
+}
+
+int32_t rblas_stream_create(void **stream) {
+  hipStream_t s;
+  RBLAS_HIP_TRY(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "hipStreamCreate");
+  *stream = (void *)s;
+  return RBLAS_OK;
+}
+
+int32_t rblas_stream_destroy(void *stream) {
+  RBLAS_HIP_TRY(hipStreamDestroy((hipStream_t)stream), "hipStreamDestroy");
+  return RBLAS_OK;
+}
+
+int32_t rblas_stream_synchronize(void *stream) {
+  RBLAS_HIP_TRY(hipStreamSynchronize((hipStream_t)stream), "hipStreamSynchronize");
+  return RBLAS_OK;
+}
+
+int32_t rblas_event_create(void **event) {
+  hipEvent_t e;
+  RBLAS_HIP_TRY(hipEventCreate(&e), "hipEventCreate");
+  *event = (void *)e;
+  return RBLAS_OK;
+}
+
+int32_t rblas_event_destroy(void *event) {
+  RBLAS_HIP_TRY(hipEventDestroy((hipEvent_t)event), "hipEventDestroy");
+  return RBLAS_OK;
+}
+
+int32_t rblas_event_record(void *event, void *stream) {
+  RBLAS_HIP_TRY(hipEventRecord((hipEvent_t)event, (hipStream_t)stream), "hipEventRecord");
+  return RBLAS_OK;
+}
+
+int32_t rblas_event_elapsed_ms(float *ms, void *start, void *stop) {
+  RBLAS_HIP_TRY(hipEventSynchronize((hipEvent_t)stop), "hipEventSynchronize");
+  RBLAS_HIP_TRY(hipEventElapsedTime(ms, (hipEvent_t)start, (hipEvent_t)stop),
+                "hipEventElapsedTime");
+  return RBLAS_OK;
+}
+
+int32_t rblas_device_synchronize(void) {
+  RBLAS_HIP_TRY(hipDeviceSynchronize(), "hipDeviceSynchronize");
+  return RBLAS_OK;
+}
+
+}  // extern "C"
