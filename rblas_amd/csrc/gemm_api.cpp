@@ -50,6 +50,16 @@ Check normalize(int32_t layout, int32_t ta, int32_t tb, size_t m, size_t n, size
     bad.msg = "gemm: m, n and k must each be below 2^31 on the device path";
     return bad;
   }
+  // A 128-wide tile of either operand must span under 2 GiB (the tile
+  // loaders address it with 31-bit byte offsets).
+  const size_t ldmax = out.lda > out.ldb ? out.lda : out.ldb;
+  const bool runs_kernel = out.m && out.n && out.k && out.alpha != T(0);
+  if (runs_kernel && (ldmax * 128 + 160) * sizeof(T) >= 0x7fffffe0ull) {
+    bad.status = RBLAS_BAD_ARG;
+    bad.err = {RBLAS_BAD_ARG, 0, 0, 0};
+    bad.msg = "gemm: lda/ldb above 2^31/(128*sizeof(T)) elements is not supported on the device";
+    return bad;
+  }
   return Check{};
 }
 

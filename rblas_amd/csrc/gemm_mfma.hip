@@ -3,23 +3,49 @@
 //
 // This replaces rblas's Goto/BLIS pipeline (kernel/gemm.rs:185-474 and the
 // AVX2 microkernels gemm_avx2.rs:26-155):
-//   - pack_a_panel / pack_b_panel  -> global -> register -> LDS tile loads,
-//     double-buffered in LDS, one barrier per k-tile;
+//   - pack_a_panel / pack_b_panel -> the buffer unit's direct global->LDS
+//     loads (`buffer_load_dwordx4 ... lds`), four LDS stages in flight, one
+//     barrier per k-tile, no staging registers;
 //   - the MR x NR register microtile -> a 64 x 64 per-wave tile of 4 x 4
 //     16 x 16 MFMA accumulators;
 //   - the jc/pc/ic loops -> one workgroup per 128 x 128 C tile, the whole k
 //     range accumulated in registers (no KC re-blocking of C);
-//   - materialize (level3.rs:179-222) -> none: each operand has a loader
-//     for its stored orientation (M/N-contiguous or K-contiguous).
+//   - materialize (level3.rs:179-222) -> none: each operand is staged from
+//     its stored orientation (M/N-contiguous or K-contiguous).
+//
+// LDS staging. One DMA instruction moves 64 lanes x 16 bytes = 64*V elements
+// (V = 16 / sizeof(T): 2 for f64, 4 for f32) into one contiguous LDS block,
+// so a 128 x kBK operand tile (kBK = 8V) is 16 blocks. Blocks are padded so
+// that the four lane groups of a fragment read fall on different banks:
+//   M/N-contiguous operand: block q holds tile k-rows q*R .. q*R+R-1 (R = V/2)
+//     as [row][128], lane l carrying V consecutive m (or n) of one row;
+//     block stride 1024 + 128/V bytes.
+//   K-contiguous operand: block q holds the 8 m (or n) q*8 .. q*8+7, each as
+//     its kBK k (128 bytes = eight 16-byte chunks of V k). Lanes 8r .. 8r+7
+//     fetch the eight chunks of one column, so each instruction reads whole
+//     cache lines; chunk c of column mn is stored in slot c ^ (mn % 8) of
+//     that column's 128 bytes, which keeps 16-lane fragment reads free of
+//     bank conflicts. Block stride 1040 bytes.
+// MFMA k-steps are assigned so that a lane's V consecutive k of the
+// K-contiguous layout feed V consecutive MFMAs: in k-step s = V*t + p the
+// lanes of group g = lane/16 use k = 4*V*t + V*g + p. Both operands follow
+// the same assignment, so every k is used exactly once.
+//
+// A partial last k-tile is staged through registers with per-element masks
+// so that every k >= K slot holds an exact zero in both operands. Rows or
+// columns past the m/n edge may hold any value: they only reach C entries
+// outside the matrix, which the epilogue does not store.
 //
 // The MFMA is fed op(B)^T as its first operand and op(A)^T as its second, so
 // the accumulator's lane index runs down a column of C: 16 consecutive lanes
-// hold 16 consecutive rows of one C column and the epilogue writes 128-byte
-// (f64) runs of column-major C.
+// hold 16 consecutive rows of one C column and the epilogue writes runs of
+// column-major C.
 #include "kernels.hpp"
 
 namespace rblas {
 namespace {
+
+typedef __attribute__((address_space(3))) void *lds_void_ptr;
 
 template <typename T>
 struct Mfma;
@@ -32,9 +58,7 @@ struct Mfma<double> {
   }
   // Accumulator element r of lane l is row (l/16) + 4r of the 16 x 16 block
   // (measured on gfx950: the f64 form interleaves rows).
-  static __device__ __forceinline__ int row(int lane, int r) {
-    return (lane >> 4) + 4 * r;
-  }
+  static __device__ __forceinline__ int row(int lane, int r) { return (lane >> 4) + 4 * r; }
 };
 
 template <>
@@ -44,79 +68,162 @@ struct Mfma<float> {
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
   }
   // Accumulator element r of lane l is row 4(l/16) + r of the 16 x 16 block.
-  static __device__ __forceinline__ int row(int lane, int r) {
-    return 4 * (lane >> 4) + r;
-  }
+  static __device__ __forceinline__ int row(int lane, int r) { return 4 * (lane >> 4) + r; }
 };
 
 constexpr int kThreads = 256;  // 4 waves of 64, 2 x 2 over the C tile
-constexpr int kBM = 128;       // C tile rows
-constexpr int kBN = 128;       // C tile columns
-constexpr int kBK = 16;        // k per LDS stage (4 MFMA k-steps)
-constexpr int kWaveM = 64;     // per-wave C tile
-constexpr int kWaveN = 64;
-constexpr int kFragM = kWaveM / 16;  // 4 MFMA blocks down
-constexpr int kFragN = kWaveN / 16;  // 4 MFMA blocks across
-constexpr int kLoads = kBM * kBK / kThreads;  // 8 elements per thread per operand
+constexpr int kTile = 128;     // C tile rows and columns
+constexpr int kWave = 64;      // per-wave C tile (rows and columns)
+constexpr int kFrag = kWave / 16;  // 4 x 4 MFMA blocks per wave
+constexpr int kStages = 4;     // LDS pipeline depth
+constexpr int kBlocks = 16;    // DMA blocks per operand tile
+constexpr int kOperandBytes = kBlocks * (1024 + 64);  // room for the largest block stride
+constexpr uint32_t kMasked = 0x80000000u;  // buffer offset past num_records: reads 0
 
 template <typename T>
-struct Pad;
-template <>
-struct Pad<double> { static constexpr int v = 1; };
-template <>
-struct Pad<float> { static constexpr int v = 4; };
+struct Geo {
+  static constexpr int V = 16 / (int)sizeof(T);   // elements per lane per DMA
+  static constexpr int BK = 8 * V;                 // k per tile: 16 (f64) or 32 (f32)
+  static constexpr int R = V / 2;                  // k-rows per block (M/N-contiguous)
+  static constexpr int NT = BK / (4 * V);          // 2 groups of V k-steps per tile
+};
 
-// One operand's k-tile: kBK x 128 elements staged as lds[k][mn].
-// MN_MAJOR: stored with the m (or n) index contiguous (A NoTrans, B Trans).
-// Otherwise stored with k contiguous (A Trans, B NoTrans).
+// Byte stride between DMA blocks of one operand tile.
 template <typename T, bool MN_MAJOR>
-struct TileLoader {
-  const T *base;  // element (mn0, 0) of this tile row/column panel
+constexpr int block_bytes() {
+  return MN_MAJOR ? 1024 + 128 / Geo<T>::V : 1024 + 16;
+}
+
+// LDS: kStages stages of (A tile, B tile), one distinct object per stage so
+// the compiler can tell the stage being read from the stages being filled.
+__shared__ __attribute__((aligned(16))) char g_lds_a0[kOperandBytes];
+__shared__ __attribute__((aligned(16))) char g_lds_a1[kOperandBytes];
+__shared__ __attribute__((aligned(16))) char g_lds_a2[kOperandBytes];
+__shared__ __attribute__((aligned(16))) char g_lds_a3[kOperandBytes];
+__shared__ __attribute__((aligned(16))) char g_lds_b0[kOperandBytes];
+__shared__ __attribute__((aligned(16))) char g_lds_b1[kOperandBytes];
+__shared__ __attribute__((aligned(16))) char g_lds_b2[kOperandBytes];
+__shared__ __attribute__((aligned(16))) char g_lds_b3[kOperandBytes];
+
+template <int S>
+__device__ __forceinline__ char *lds_a() {
+  if constexpr (S == 0) return g_lds_a0;
+  else if constexpr (S == 1) return g_lds_a1;
+  else if constexpr (S == 2) return g_lds_a2;
+  else return g_lds_a3;
+}
+template <int S>
+__device__ __forceinline__ char *lds_b() {
+  if constexpr (S == 0) return g_lds_b0;
+  else if constexpr (S == 1) return g_lds_b1;
+  else if constexpr (S == 2) return g_lds_b2;
+  else return g_lds_b3;
+}
+
+// Raw buffer resource over [base, base + bytes), bytes clamped to 2^31 - 1 so
+// that kMasked is always out of range. The host guarantees every in-tile
+// offset is below that (gemm_fits).
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void *base, uint64_t bytes) {
+  const int n = bytes >= 0x7fffffffull ? 0x7fffffff : (int)bytes;
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), (short)0, n, 0x00020000);
+}
+
+template <typename T>
+__device__ __forceinline__ T buffer_load(__amdgpu_buffer_rsrc_t r, uint32_t off);
+template <>
+__device__ __forceinline__ double buffer_load<double>(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+  return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
+}
+template <>
+__device__ __forceinline__ float buffer_load<float>(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
+}
+
+// Byte offset of tile element (k, mn) inside one operand's LDS tile.
+template <typename T, bool MN_MAJOR>
+__device__ __forceinline__ int lds_byte(int k, int mn) {
+  typedef Geo<T> G;
+  constexpr int bb = block_bytes<T, MN_MAJOR>();
+  if (MN_MAJOR) return (k / G::R) * bb + ((k % G::R) * kTile + mn) * (int)sizeof(T);
+  const int r = mn % 8;
+  return (mn / 8) * bb + r * 128 + (((k / G::V) ^ r) * 16) + (k % G::V) * (int)sizeof(T);
+}
+
+// One operand (op(A) as m x k or op(B)^T as n x k) of the tile loop.
+template <typename T, bool MN_MAJOR>
+struct Operand {
+  typedef Geo<T> G;
+  const T *x;
+  const T *end;  // one past the last element the stored operand may touch
   int64_t ld;
-  int mn_lim;     // valid mn extent inside this tile (<= 128)
-  int idx_mn, idx_k;  // this thread's first (mn, k) in the tile
+  int mn0, lim;     // tile origin along m/n, valid extent along m/n
+  uint32_t off[4];  // this lane's byte offset for each of its wave's 4 DMA blocks
+  int block0;       // first DMA block owned by this wave
 
-  __device__ __forceinline__ void init(const T *x, int64_t ld_, int mn0, int mn_total,
-                                       int tid) {
+  __device__ __forceinline__ void init(const T *x_, const T *end_, int64_t ld_, int mn0_,
+                                       int mn_total, int wave, int lane) {
+    x = x_;
+    end = end_;
     ld = ld_;
-    mn_lim = mn_total - mn0;
-    if (MN_MAJOR) {
-      idx_mn = tid % kBM;
-      idx_k = tid / kBM;  // 0..1, stepped by 2
-      base = x + mn0;
-    } else {
-      idx_k = tid % kBK;
-      idx_mn = tid / kBK;  // 0..15, stepped by 16
-      base = x + (int64_t)mn0 * ld;
-    }
-  }
-
-  // Load k-tile starting at k0 into registers; out-of-range elements are 0.
-  __device__ __forceinline__ void load(T (&r)[kLoads], int k0, int k_total) const {
+    mn0 = mn0_;
+    lim = mn_total - mn0_;
+    block0 = wave * 4;
 #pragma unroll
-    for (int s = 0; s < kLoads; ++s) {
-      int mn, k;
+    for (int j = 0; j < 4; ++j) {
+      const int q = block0 + j;
+      int k, mn;
       if (MN_MAJOR) {
-        mn = idx_mn;
-        k = idx_k + 2 * s;
+        const int e = lane * G::V;
+        k = q * G::R + e / kTile;
+        mn = e % kTile;
       } else {
-        mn = idx_mn + 16 * s;
-        k = idx_k;
+        const int r = lane / 8;            // column within the block
+        k = ((lane % 8) ^ r) * G::V;       // chunk stored in slot lane % 8
+        mn = q * 8 + r;
       }
-      const int kg = k0 + k;
-      const bool ok = (mn < mn_lim) && (kg < k_total);
-      const T *src = MN_MAJOR ? base + mn + (int64_t)kg * ld : base + kg + (int64_t)mn * ld;
-      r[s] = ok ? *src : T(0);
+      const int64_t el = MN_MAJOR ? mn + (int64_t)k * ld : k + (int64_t)mn * ld;
+      off[j] = (uint32_t)(el * (int64_t)sizeof(T));
     }
   }
 
-  __device__ __forceinline__ void store(const T (&r)[kLoads], T (*lds)[kBM + Pad<T>::v]) const {
+  __device__ __forceinline__ const T *origin(int k0) const {
+    return MN_MAJOR ? x + mn0 + (int64_t)k0 * ld : x + k0 + (int64_t)mn0 * ld;
+  }
+
+  // Full k-tile: 4 DMA blocks per wave straight into LDS.
+  __device__ __forceinline__ void dma(char *lds, int k0) const {
+    const T *o = origin(k0);
+    const __amdgpu_buffer_rsrc_t rs =
+        make_rsrc(o, (uint64_t)((const char *)end - (const char *)o));
 #pragma unroll
-    for (int s = 0; s < kLoads; ++s) {
-      if (MN_MAJOR)
-        lds[idx_k + 2 * s][idx_mn] = r[s];
-      else
-        lds[idx_k][idx_mn + 16 * s] = r[s];
+    for (int j = 0; j < 4; ++j)
+      __builtin_amdgcn_raw_ptr_buffer_load_lds(
+          rs, (lds_void_ptr)(lds + (block0 + j) * block_bytes<T, MN_MAJOR>()), 16, off[j], 0, 0,
+          0);
+  }
+
+  // Partial last k-tile: element-wise masked loads, written in the same
+  // layout; k >= K and m/n past the edge store exact zeros.
+  __device__ __forceinline__ void masked(char *lds, int k0, int K, int tid) const {
+    const T *o = origin(k0);
+    const __amdgpu_buffer_rsrc_t rs =
+        make_rsrc(o, (uint64_t)((const char *)end - (const char *)o));
+    const int krem = K - k0;
+#pragma unroll
+    for (int s = 0; s < kTile * G::BK / kThreads; ++s) {
+      const int e = tid + kThreads * s;
+      int k, mn;
+      if (MN_MAJOR) {
+        mn = e % kTile;
+        k = e / kTile;
+      } else {
+        k = e % G::BK;
+        mn = e / G::BK;
+      }
+      const int64_t el = MN_MAJOR ? mn + (int64_t)k * ld : k + (int64_t)mn * ld;
+      const bool ok = k < krem && mn < lim;
+      *(T *)(lds + lds_byte<T, MN_MAJOR>(k, mn)) =
+          buffer_load<T>(rs, ok ? (uint32_t)(el * (int64_t)sizeof(T)) : kMasked);
     }
   }
 };
@@ -141,73 +248,167 @@ __device__ __forceinline__ void tile_coords(int tiles_m, int tiles_n, int &tm, i
   tn = in_group / gm;
 }
 
+// Workgroup barrier for the LDS pipeline: wait until at most N of this
+// wave's vector-memory instructions (the DMA loads of later tiles) are
+// outstanding and all its LDS stores are done, then s_barrier. Unlike
+// __syncthreads(), whose release fence drains every outstanding load
+// (vmcnt(0)), this keeps the later stages in flight. The empty asm statements
+// stop the compiler from moving memory accesses across the barrier.
+template <int N>
+__device__ __forceinline__ void stage_barrier() {
+  static_assert(N >= 0 && N < 64, "vmcnt range");
+  asm volatile("" ::: "memory");
+  __builtin_amdgcn_s_waitcnt((N & 0xf) | ((N >> 4) << 14) | 0x0070);  // vmcnt(N) lgkmcnt(0)
+  __builtin_amdgcn_s_barrier();
+  asm volatile("" ::: "memory");
+}
+
+template <typename T, bool TA, bool TB>
+struct TileGemm {
+  typedef Geo<T> G;
+  typedef typename Mfma<T>::acc_t acc_t;
+  static constexpr bool A_MN = !TA;   // A NoTrans is M-contiguous
+  static constexpr bool B_MN = TB;    // B Trans is N-contiguous
+  static constexpr int kPerTile = 8;  // DMA instructions per wave per k-tile
+
+  Operand<T, A_MN> a;
+  Operand<T, B_MN> b;
+  acc_t acc[kFrag][kFrag];
+  int K, ktiles, full_tiles, tid, lane, wm, wn;
+
+  // Stage k-tile kt into LDS stage S (DMA when full, masked loads otherwise).
+  template <int S>
+  __device__ __forceinline__ void issue(int kt) {
+    if (kt >= ktiles) return;
+    if (kt < full_tiles) {
+      a.dma(lds_a<S>(), kt * G::BK);
+      b.dma(lds_b<S>(), kt * G::BK);
+    } else {
+      a.masked(lds_a<S>(), kt * G::BK, K, tid);
+      b.masked(lds_b<S>(), kt * G::BK, K, tid);
+    }
+  }
+
+  // Barrier after which k-tile kt is in LDS for every wave. The tiles issued
+  // after kt are kt+1 .. kt+ahead, kPerTile DMA instructions per wave each;
+  // vector memory instructions complete in order, so vmcnt(kPerTile * ahead)
+  // means kt's have landed. (If one of the later tiles was the masked tail,
+  // its register loads were consumed by its own LDS stores, which already
+  // waited for everything issued before them.)
+  __device__ __forceinline__ void sync_tile(int kt) {
+    const int ahead = min(kStages - 2, ktiles - 1 - kt);
+    if (ahead >= 2)
+      stage_barrier<2 * kPerTile>();
+    else if (ahead == 1)
+      stage_barrier<kPerTile>();
+    else
+      stage_barrier<0>();
+  }
+
+  template <int S>
+  __device__ __forceinline__ void compute() {
+    const char *As = lds_a<S>();
+    const char *Bs = lds_b<S>();
+    const int g = lane >> 4, c = lane & 15;
+    typedef T vec_t __attribute__((ext_vector_type(G::V)));
+#pragma unroll
+    for (int t = 0; t < G::NT; ++t) {
+      T fa[G::V][kFrag], fb[G::V][kFrag];
+      const int kb = 4 * G::V * t + G::V * g;  // this lane group's first k
+#pragma unroll
+      for (int i = 0; i < kFrag; ++i) {
+        const int mn = wm + 16 * i + c;
+        if constexpr (A_MN) {
+#pragma unroll
+          for (int p = 0; p < G::V; ++p) fa[p][i] = *(const T *)(As + lds_byte<T, true>(kb + p, mn));
+        } else {
+          const vec_t v = *(const vec_t *)(As + lds_byte<T, false>(kb, mn));
+#pragma unroll
+          for (int p = 0; p < G::V; ++p) fa[p][i] = v[p];
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < kFrag; ++j) {
+        const int mn = wn + 16 * j + c;
+        if constexpr (B_MN) {
+#pragma unroll
+          for (int p = 0; p < G::V; ++p) fb[p][j] = *(const T *)(Bs + lds_byte<T, true>(kb + p, mn));
+        } else {
+          const vec_t v = *(const vec_t *)(Bs + lds_byte<T, false>(kb, mn));
+#pragma unroll
+          for (int p = 0; p < G::V; ++p) fb[p][j] = v[p];
+        }
+      }
+#pragma unroll
+      for (int p = 0; p < G::V; ++p)
+#pragma unroll
+        for (int i = 0; i < kFrag; ++i)
+#pragma unroll
+          for (int j = 0; j < kFrag; ++j)
+            acc[i][j] = Mfma<T>::mma(fb[p][j], fa[p][i], acc[i][j]);
+    }
+  }
+
+  // Pipeline step for k-tile kt, which lives in stage S.
+  template <int S>
+  __device__ __forceinline__ void step(int kt) {
+    if (kt >= ktiles) return;
+    sync_tile(kt);  // tile kt visible to all waves; the stage of kt-1 is free
+    issue<(S + kStages - 1) % kStages>(kt + kStages - 1);
+    compute<S>();
+  }
+};
+
+// One past the last element a stored rows x cols column-major operand with
+// leading dimension ld can touch.
+template <typename T>
+__device__ __forceinline__ const T *extent_end(const T *x, int64_t rows, int64_t cols,
+                                               int64_t ld) {
+  return x + (cols - 1) * ld + rows;
+}
+
 template <typename T, bool TA, bool TB>
 __global__ __launch_bounds__(kThreads) void gemm_tile_kernel(GemmProblem<T> p) {
+  typedef Geo<T> G;
   typedef typename Mfma<T>::acc_t acc_t;
-  constexpr int P = Pad<T>::v;
-  __shared__ T As[2][kBK][kBM + P];
-  __shared__ T Bs[2][kBK][kBN + P];
+  TileGemm<T, TA, TB> tg;
 
   const int M = (int)p.m, N = (int)p.n, K = (int)p.k;
-  const int tiles_m = (M + kBM - 1) / kBM;
-  const int tiles_n = (N + kBN - 1) / kBN;
+  const int tiles_m = (M + kTile - 1) / kTile;
+  const int tiles_n = (N + kTile - 1) / kTile;
   int tm, tn;
   tile_coords(tiles_m, tiles_n, tm, tn);
-  const int m0 = tm * kBM, n0 = tn * kBN;
+  const int m0 = tm * kTile, n0 = tn * kTile;
 
-  const int tid = threadIdx.x;
-  const int lane = tid & 63;
-  const int wave = tid >> 6;
-  const int wm = (wave & 1) * kWaveM;
-  const int wn = (wave >> 1) * kWaveN;
+  tg.tid = threadIdx.x;
+  tg.lane = tg.tid & 63;
+  const int wave = tg.tid >> 6;
+  tg.wm = (wave & 1) * kWave;
+  tg.wn = (wave >> 1) * kWave;
+  tg.K = K;
+  tg.ktiles = (K + G::BK - 1) / G::BK;
+  tg.full_tiles = K / G::BK;
 
-  // A NoTrans is M-contiguous; B Trans is N-contiguous.
-  TileLoader<T, !TA> la;
-  TileLoader<T, TB> lb;
-  la.init(p.a, p.lda, m0, M, tid);
-  lb.init(p.b, p.ldb, n0, N, tid);
+  // A NoTrans is stored m x k; A Trans is stored k x m.
+  // B NoTrans is stored k x n; B Trans is stored n x k.
+  tg.a.init(p.a, TA ? extent_end(p.a, p.k, p.m, p.lda) : extent_end(p.a, p.m, p.k, p.lda), p.lda,
+            m0, M, wave, tg.lane);
+  tg.b.init(p.b, TB ? extent_end(p.b, p.n, p.k, p.ldb) : extent_end(p.b, p.k, p.n, p.ldb), p.ldb,
+            n0, N, wave, tg.lane);
 
-  acc_t acc[kFragM][kFragN];
 #pragma unroll
-  for (int i = 0; i < kFragM; ++i)
+  for (int i = 0; i < kFrag; ++i)
 #pragma unroll
-    for (int j = 0; j < kFragN; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
+    for (int j = 0; j < kFrag; ++j) tg.acc[i][j] = acc_t{0, 0, 0, 0};
 
-  T ra[kLoads], rb[kLoads];
-  const int ktiles = (K + kBK - 1) / kBK;
-  la.load(ra, 0, K);
-  lb.load(rb, 0, K);
-  la.store(ra, As[0]);
-  lb.store(rb, Bs[0]);
-  __syncthreads();
-
-  const int fk = lane >> 4;   // k row of this lane's fragment within a k-step
-  const int fmn = lane & 15;  // m/n column of this lane's fragment
-
-  for (int kt = 0; kt < ktiles; ++kt) {
-    const int cur = kt & 1;
-    const bool more = kt + 1 < ktiles;
-    if (more) {
-      la.load(ra, (kt + 1) * kBK, K);
-      lb.load(rb, (kt + 1) * kBK, K);
-    }
-#pragma unroll
-    for (int ks = 0; ks < kBK; ks += 4) {
-      T fa[kFragM], fb[kFragN];
-#pragma unroll
-      for (int i = 0; i < kFragM; ++i) fa[i] = As[cur][ks + fk][wm + 16 * i + fmn];
-#pragma unroll
-      for (int j = 0; j < kFragN; ++j) fb[j] = Bs[cur][ks + fk][wn + 16 * j + fmn];
-#pragma unroll
-      for (int i = 0; i < kFragM; ++i)
-#pragma unroll
-        for (int j = 0; j < kFragN; ++j) acc[i][j] = Mfma<T>::mma(fb[j], fa[i], acc[i][j]);
-    }
-    if (more) {
-      la.store(ra, As[cur ^ 1]);
-      lb.store(rb, Bs[cur ^ 1]);
-    }
-    __syncthreads();
+  tg.template issue<0>(0);
+  tg.template issue<1>(1);
+  tg.template issue<2>(2);
+  for (int kt = 0; kt < tg.ktiles; kt += kStages) {
+    tg.template step<0>(kt);
+    tg.template step<1>(kt + 1);
+    tg.template step<2>(kt + 2);
+    tg.template step<3>(kt + 3);
   }
 
   // Epilogue: element (i, j) of C gets alpha*acc + beta*C; beta == 0 never
@@ -215,18 +416,19 @@ __global__ __launch_bounds__(kThreads) void gemm_tile_kernel(GemmProblem<T> p) {
   // columns j = n0+wn+16fj+row(l, r).
   const T alpha = p.alpha, beta = p.beta;
   const bool read_c = beta != T(0);
+  const int c16 = tg.lane & 15;
 #pragma unroll
-  for (int i = 0; i < kFragM; ++i) {
-    const int gi = m0 + wm + 16 * i + fmn;
+  for (int i = 0; i < kFrag; ++i) {
+    const int gi = m0 + tg.wm + 16 * i + c16;
     if (gi >= M) continue;
 #pragma unroll
-    for (int j = 0; j < kFragN; ++j) {
+    for (int j = 0; j < kFrag; ++j) {
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int gj = n0 + wn + 16 * j + Mfma<T>::row(lane, r);
+        const int gj = n0 + tg.wn + 16 * j + Mfma<T>::row(tg.lane, r);
         if (gj >= N) continue;
         T *cp = p.c + gi + (int64_t)gj * p.ldc;
-        const T v = acc[i][j][r];
+        const T v = tg.acc[i][j][r];
         *cp = read_c ? __builtin_fma(alpha, v, beta * *cp) : alpha * v;
       }
     }
@@ -247,8 +449,17 @@ __global__ void scale_kernel(int64_t m, int64_t n, T beta, T *c, int64_t ldc) {
 }  // namespace
 
 template <typename T>
+bool gemm_fits(const GemmProblem<T> &p) {
+  // Largest in-tile byte offset for either operand orientation.
+  const int64_t ld = p.lda > p.ldb ? p.lda : p.ldb;
+  const int64_t span = (int64_t)(kTile - 1) * ld + kTile + Geo<T>::BK;
+  return span * (int64_t)sizeof(T) < 0x7fffffffll - 16;
+}
+
+template <typename T>
 hipError_t launch_gemm(const GemmProblem<T> &p, hipStream_t stream) {
-  const int64_t tiles = ((p.m + kBM - 1) / kBM) * ((p.n + kBN - 1) / kBN);
+  if (!gemm_fits(p)) return hipErrorInvalidValue;
+  const int64_t tiles = ((p.m + kTile - 1) / kTile) * ((p.n + kTile - 1) / kTile);
   dim3 grid((unsigned)tiles), block(kThreads);
   if (!p.ta && !p.tb)
     hipLaunchKernelGGL((gemm_tile_kernel<T, false, false>), grid, block, 0, stream, p);
