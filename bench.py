@@ -39,7 +39,36 @@ def parse():
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--extra", action="store_true",
                    help="also time SGEMM (plain FP32) at the same size")
+    p.add_argument("--copy-calibrate", action="store_true",
+                   help="only run 5 device-to-device copies of 1 GiB (counter calibration)")
     return p.parse_args()
+
+
+PROFILE = os.path.join(ROOT, "profiles", "dgemm_8192_nn.md")
+
+
+def measured_traffic():
+    """HBM bytes per gemm launch from the committed rocprofv3 PMC summary
+    (FETCH_SIZE + WRITE_SIZE, scaled by the copy calibration), or None."""
+    try:
+        for line in open(PROFILE):
+            if line.startswith("traffic_bytes_per_launch:"):
+                return int(float(line.split(":", 1)[1].strip()))
+    except OSError:
+        pass
+    return None
+
+
+def copy_calibrate():
+    from rblas_amd import _ffi
+    from rblas_amd.device import DeviceArray, Stream
+    n = 1 << 30
+    s = Stream()
+    src, dst = DeviceArray(n, np.uint8), DeviceArray(n, np.uint8)
+    for _ in range(5):
+        _ffi.lib().rblas_memcpy_dtod(dst.ptr, src.ptr, n, s.handle)
+    s.synchronize()
+    print(json.dumps({"copy_calibration_bytes": n, "copies": 5}), flush=True)
 
 
 def gen_operands(n, ld, dtype, seed=1):
@@ -102,6 +131,9 @@ def cpu_baseline(n, ld):
 
 def main():
     args = parse()
+    if args.copy_calibrate:
+        copy_calibrate()
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
                           "(multi-GPU sharding is described in DESIGN.md, not built)"}))
@@ -128,7 +160,8 @@ def main():
                    "pct_fp64_peak": round(100 * tflops / FP64_MFMA_PEAK_TFLOPS, 2)},
         "roofline": {"bound": "mfma", "achieved": round(tflops, 3),
                      "peak": FP64_MFMA_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": round(tflops / FP64_MFMA_PEAK_TFLOPS, 4), "traffic": None},
+                     "frac": round(tflops / FP64_MFMA_PEAK_TFLOPS, 4),
+                     "traffic": measured_traffic()},
     }
     if args.extra:
         ms32 = time_gemm(np.float32, n, ld, args.steps, args.warmup)
