@@ -124,6 +124,31 @@ int32_t rblas_dgemm_dev(int32_t layout, int32_t transa, int32_t transb,
                         double beta, double *c, size_t c_len, size_t ldc,
                         void *stream);
 
+/* SGEMM with a selectable arithmetic (no rblas counterpart: rblas's SGEMM
+ * is plain f32). `compute` is one of:
+ *   RBLAS_COMPUTE_F32     plain f32 MFMA (what rblas_sgemm does)
+ *   RBLAS_COMPUTE_BF16X3  each f32 split into 2 bf16 terms, 3 cross products
+ *                         on bf16 MFMA with f32 accumulation
+ *   RBLAS_COMPUTE_BF16X6  3 bf16 terms, 6 cross products (about f32 accuracy)
+ * Arguments and validation otherwise as rblas_sgemm / rblas_sgemm_dev. The
+ * split forms need a device workspace of 2 or 3 bf16 copies of op(A) and
+ * op(B) (rounded up to 256 rows and 48 k, plus 32 k); the `_dev` form takes it from the
+ * HIP stream-ordered pool of the current device. Non-finite inputs and
+ * magnitudes above 3.38e38 (bf16's largest finite) are outside the split
+ * forms' domain. */
+enum { RBLAS_COMPUTE_F32 = 0, RBLAS_COMPUTE_BF16X3 = 1, RBLAS_COMPUTE_BF16X6 = 2 };
+int32_t rblas_sgemm_ex(int32_t layout, int32_t transa, int32_t transb, size_t m,
+                       size_t n, size_t k, float alpha, const float *a,
+                       size_t a_len, size_t lda, const float *b, size_t b_len,
+                       size_t ldb, float beta, float *c, size_t c_len,
+                       size_t ldc, int32_t compute);
+int32_t rblas_sgemm_ex_dev(int32_t layout, int32_t transa, int32_t transb,
+                           size_t m, size_t n, size_t k, float alpha,
+                           const float *a, size_t a_len, size_t lda,
+                           const float *b, size_t b_len, size_t ldb,
+                           float beta, float *c, size_t c_len, size_t ldc,
+                           int32_t compute, void *stream);
+
 /* ------------------------------------------------- device memory & streams
  * Plumbing for callers that keep operands resident in HBM (no rblas
  * counterpart). All return a status. */

@@ -42,6 +42,8 @@ SIGNATURES: dict[str, tuple[list, object]] = {
     "rblas_sgemm_try": (_gemm_args(f32) + [C.POINTER(RblasError)], i32),
     "rblas_dgemm_try": (_gemm_args(f64) + [C.POINTER(RblasError)], i32),
     "rblas_sgemm_dev": (_gemm_args(f32) + [vp], i32),
+    "rblas_sgemm_ex": (_gemm_args(f32) + [i32], i32),
+    "rblas_sgemm_ex_dev": (_gemm_args(f32) + [i32, vp], i32),
     "rblas_dgemm_dev": (_gemm_args(f64) + [vp], i32),
     "rblas_malloc": ([C.POINTER(vp), sz], i32),
     "rblas_free": ([vp], i32),

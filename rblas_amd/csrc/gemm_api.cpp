@@ -63,12 +63,42 @@ Check normalize(int32_t layout, int32_t ta, int32_t tb, size_t m, size_t n, size
   return Check{};
 }
 
+// Whether a normalized call reaches the product kernel (else it is a no-op
+// or a beta-only update).
+template <typename T>
+bool runs_product(const ColMajorCall<T> &cc) {
+  return cc.m != 0 && cc.n != 0 && cc.k != 0 && cc.alpha != T(0);
+}
+
+// Split planes for an SGEMM compute mode (0 = plain f32).
+inline int split_planes(int32_t compute) {
+  return compute == RBLAS_COMPUTE_BF16X3 ? 2 : compute == RBLAS_COMPUTE_BF16X6 ? 3 : 0;
+}
+
+template <typename T>
+size_t workspace_bytes(const ColMajorCall<T> &cc, int planes) {
+  if (planes == 0 || !runs_product(cc)) return 0;
+  return sgemm_split_workspace((int64_t)cc.m, (int64_t)cc.n, (int64_t)cc.k, planes);
+}
+
+template <typename T>
+hipError_t launch_product(const GemmProblem<T> &p, int planes, void *ws, hipStream_t st) {
+  return launch_gemm<T>(p, st);
+}
+template <>
+hipError_t launch_product<float>(const GemmProblem<float> &p, int planes, void *ws,
+                                 hipStream_t st) {
+  if (planes) return launch_sgemm_split(p, planes, ws, st);
+  return launch_gemm<float>(p, st);
+}
+
 // Enqueue the device work of a normalized call whose pointers are in HBM.
 // Mirrors gemm_contig's control flow (kernel/gemm.rs:198-205): nothing when
-// m or n is 0; beta-only update when k is 0 or alpha is 0.
+// m or n is 0; beta-only update when k is 0 or alpha is 0. `ws` is the split
+// workspace (planes > 0 only).
 template <typename T>
 hipError_t enqueue(const ColMajorCall<T> &cc, const T *a, size_t lda, const T *b, size_t ldb, T *c,
-                   size_t ldc, hipStream_t st) {
+                   size_t ldc, int planes, void *ws, hipStream_t st) {
   if (cc.m == 0 || cc.n == 0) return hipSuccess;
   if (cc.k == 0 || cc.alpha == T(0)) {
     if (cc.beta == T(1)) return hipSuccess;
@@ -76,11 +106,11 @@ hipError_t enqueue(const ColMajorCall<T> &cc, const T *a, size_t lda, const T *b
   }
   GemmProblem<T> p{cc.at, cc.bt, (int64_t)cc.m, (int64_t)cc.n, (int64_t)cc.k, cc.alpha, cc.beta,
                    a, (int64_t)lda, b, (int64_t)ldb, c, (int64_t)ldc};
-  return launch_gemm<T>(p, st);
+  return launch_product<T>(p, planes, ws, st);
 }
 
 template <typename T>
-int32_t gemm_host(const ColMajorCall<T> &cc) {
+int32_t gemm_host(const ColMajorCall<T> &cc, int planes = 0) {
   if (cc.m == 0 || cc.n == 0) return RBLAS_OK;
   CtxLease lease;
   if (!lease.get()) return hip_fail(lease.status(), "stream create");
@@ -95,6 +125,7 @@ int32_t gemm_host(const ColMajorCall<T> &cc) {
   RBLAS_HIP_TRY(ctx.ensure(0, sa.elems * sizeof(T)), "device buffer A");
   RBLAS_HIP_TRY(ctx.ensure(1, sb.elems * sizeof(T)), "device buffer B");
   RBLAS_HIP_TRY(ctx.ensure(2, sc.elems * sizeof(T)), "device buffer C");
+  RBLAS_HIP_TRY(ctx.ensure(3, workspace_bytes(cc, planes)), "device split workspace");
   T *da = (T *)ctx.buf[0], *db = (T *)ctx.buf[1], *dc = (T *)ctx.buf[2];
   hipStream_t st = ctx.stream;
   if (need_ab) {
@@ -102,7 +133,8 @@ int32_t gemm_host(const ColMajorCall<T> &cc) {
     RBLAS_HIP_TRY(stage_in(sb, cc.b, db, st), "copy B to device");
   }
   RBLAS_HIP_TRY(stage_in(sc, (const T *)cc.c, dc, st), "copy C to device");
-  RBLAS_HIP_TRY(enqueue(cc, da, sa.dev_ld, db, sb.dev_ld, dc, sc.dev_ld, st), "gemm launch");
+  RBLAS_HIP_TRY(enqueue(cc, da, sa.dev_ld, db, sb.dev_ld, dc, sc.dev_ld, planes, ctx.buf[3], st),
+                "gemm launch");
   RBLAS_HIP_TRY(stage_out(sc, dc, cc.c, st), "copy C to host");
   RBLAS_HIP_TRY(hipStreamSynchronize(st), "gemm");
   return RBLAS_OK;
@@ -143,14 +175,32 @@ int32_t gemm_try_entry(int32_t layout, int32_t ta, int32_t tb, size_t m, size_t 
 template <typename T>
 int32_t gemm_dev_entry(int32_t layout, int32_t ta, int32_t tb, size_t m, size_t n, size_t k,
                        T alpha, const T *a, size_t a_len, size_t lda, const T *b, size_t b_len,
-                       size_t ldb, T beta, T *c, size_t c_len, size_t ldc, void *stream) {
+                       size_t ldb, T beta, T *c, size_t c_len, size_t ldc, void *stream,
+                       int planes = 0) {
   ColMajorCall<T> cc;
   Check chk = normalize(layout, ta, tb, m, n, k, alpha, a, a_len, lda, b, b_len, ldb, beta, c,
                         c_len, ldc, cc);
   if (!chk.ok()) return fail(chk, nullptr);
-  RBLAS_HIP_TRY(enqueue(cc, cc.a, cc.lda, cc.b, cc.ldb, cc.c, cc.ldc, (hipStream_t)stream),
-                "gemm launch");
+  hipStream_t st = (hipStream_t)stream;
+  const size_t wsb = workspace_bytes(cc, planes);
+  void *ws = nullptr;
+  if (wsb) RBLAS_HIP_TRY(pool_alloc(&ws, wsb, st), "split workspace");
+  hipError_t e = enqueue(cc, cc.a, cc.lda, cc.b, cc.ldb, cc.c, cc.ldc, planes, ws, st);
+  if (ws) hipFreeAsync(ws, st);
+  RBLAS_HIP_TRY(e, "gemm launch");
   return RBLAS_OK;
+}
+
+Check bad_compute() {
+  Check bad;
+  bad.status = RBLAS_BAD_ENUM;
+  bad.err = {RBLAS_BAD_ENUM, 0, 0, 0};
+  bad.msg = "sgemm_ex: compute mode out of range";
+  return bad;
+}
+
+inline bool valid_compute(int32_t c) {
+  return c == RBLAS_COMPUTE_F32 || c == RBLAS_COMPUTE_BF16X3 || c == RBLAS_COMPUTE_BF16X6;
 }
 
 }  // namespace
@@ -188,6 +238,27 @@ int32_t rblas_dgemm_try(int32_t layout, int32_t transa, int32_t transb, size_t m
                         size_t c_len, size_t ldc, rblas_error_t *err) {
   return gemm_try_entry<double>(layout, transa, transb, m, n, k, alpha, a, a_len, lda, b, b_len,
                                 ldb, beta, c, c_len, ldc, err);
+}
+
+int32_t rblas_sgemm_ex(int32_t layout, int32_t transa, int32_t transb, size_t m, size_t n,
+                       size_t k, float alpha, const float *a, size_t a_len, size_t lda,
+                       const float *b, size_t b_len, size_t ldb, float beta, float *c,
+                       size_t c_len, size_t ldc, int32_t compute) {
+  if (!valid_compute(compute)) return fail(bad_compute(), nullptr);
+  ColMajorCall<float> cc;
+  Check chk = normalize(layout, transa, transb, m, n, k, alpha, a, a_len, lda, b, b_len, ldb,
+                        beta, c, c_len, ldc, cc);
+  if (!chk.ok()) return fail(chk, nullptr);
+  return gemm_host(cc, split_planes(compute));
+}
+
+int32_t rblas_sgemm_ex_dev(int32_t layout, int32_t transa, int32_t transb, size_t m, size_t n,
+                           size_t k, float alpha, const float *a, size_t a_len, size_t lda,
+                           const float *b, size_t b_len, size_t ldb, float beta, float *c,
+                           size_t c_len, size_t ldc, int32_t compute, void *stream) {
+  if (!valid_compute(compute)) return fail(bad_compute(), nullptr);
+  return gemm_dev_entry<float>(layout, transa, transb, m, n, k, alpha, a, a_len, lda, b, b_len,
+                               ldb, beta, c, c_len, ldc, stream, split_planes(compute));
 }
 
 int32_t rblas_sgemm_dev(int32_t layout, int32_t transa, int32_t transb, size_t m, size_t n,

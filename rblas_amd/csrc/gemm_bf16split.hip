@@ -1,0 +1,367 @@
+// Split-precision SGEMM on bf16 MFMA (gfx950): each f32 operand element x is
+// written as a sum of bfloat16 terms and C = alpha * op(A) * op(B) + beta * C
+// is accumulated in f32 from the cross products of those terms.
+//
+//   BF16x3: x = h + l            (2 planes), products h*h + h*l + l*h
+//   BF16x6: x = h + m + l        (3 planes), products h*h + h*m + m*h
+//                                             + h*l + m*m + l*h
+// (the split and product sets of Henry, Tang and Heinecke, "Leveraging the
+// bfloat16 artificial intelligence datatype for higher-precision
+// computations", ARITH 2019). Each term is the round-to-nearest bf16 of the
+// remainder left by the previous ones; in f32 those remainders are exact, so
+// h + m + l represents x to about 2^-24 relative and h + l to about 2^-16.
+//
+// Two kernels:
+//   split_kernel   reads op(A) (m x k) or op(B)^T (n x k) from the caller's
+//                  layout, transposition and leading dimension and writes the
+//                  planes k-blocked by 16, zero-padded to 256 rows and to a
+//                  multiple of 48 k, so the product kernel needs no masking;
+//   split_gemm_kernel  256 x 256 C tile per workgroup, 4 waves of 128 x 128 as
+//                  4 x 4 v_mfma_f32_32x32x16_bf16 blocks (256 accumulator
+//                  registers); every plane's 16-k slice is moved global -> LDS
+//                  by buffer_load_dwordx4 ... lds (3 LDS stages, one barrier per
+//                  16 k) and each fragment is reused in registers by every
+//                  product it takes part in (96 MFMAs per barrier for BF16x6).
+#include "kernels.hpp"
+
+namespace rblas {
+namespace {
+
+typedef __attribute__((address_space(3))) void *lds_void_ptr;
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+constexpr int kThreads = 256;
+constexpr int kTile = 256;             // C tile rows and columns; plane row padding
+constexpr int kWave = 128;             // per-wave C tile: 4 x 4 blocks of 32 x 32
+constexpr int kBK = 16;                // k per LDS stage (one MFMA k-step); plane k padding
+constexpr int kStages = 3;
+constexpr int kPlaneTileBytes = kTile * kBK * 2;  // 8 KB: 256 rows x 32 B
+constexpr int kMaxPlanes = 3;
+constexpr int kOperandBytes = kMaxPlanes * kPlaneTileBytes;
+
+__shared__ __attribute__((aligned(16))) char g_sa0[kOperandBytes];
+__shared__ __attribute__((aligned(16))) char g_sa1[kOperandBytes];
+__shared__ __attribute__((aligned(16))) char g_sa2[kOperandBytes];
+__shared__ __attribute__((aligned(16))) char g_sb0[kOperandBytes];
+__shared__ __attribute__((aligned(16))) char g_sb1[kOperandBytes];
+__shared__ __attribute__((aligned(16))) char g_sb2[kOperandBytes];
+
+template <int S>
+__device__ __forceinline__ char *lds_a() {
+  if constexpr (S == 0) return g_sa0;
+  else if constexpr (S == 1) return g_sa1;
+  else return g_sa2;
+}
+template <int S>
+__device__ __forceinline__ char *lds_b() {
+  if constexpr (S == 0) return g_sb0;
+  else if constexpr (S == 1) return g_sb1;
+  else return g_sb2;
+}
+
+// LDS byte offset of plane s, tile row r (0..255), 16-byte k-chunk c (0..1):
+// rows are 32 B, so 64 lanes reading rows r..r+31 at chunks 0 and 1 read one
+// contiguous, bank-conflict-free kilobyte, in exactly the order the DMA
+// writes it.
+__device__ __forceinline__ int lds_off(int s, int r, int c) {
+  return s * kPlaneTileBytes + r * 32 + c * 16;
+}
+
+// ----------------------------------------------------------------- split
+// Source element (r, k) of op(X) (r < rows, k < K) is x[r + k*ld] when
+// R_CONTIG (A NoTrans, B Trans) and x[k + r*ld] otherwise. Plane s stores
+// the padded rows_pad x Kp matrix k-blocked: element (r, k) at
+//   planes + s*plane_elems + ((k/16)*rows_pad + r)*16 + k%16,
+// so the 16-k slice of any run of rows is contiguous (32 B per row), and a
+// product-kernel stage reads whole cache lines. Padding is zero.
+template <bool R_CONTIG, int PLANES>
+__global__ __launch_bounds__(256) void split_kernel(const float *__restrict__ x, int64_t ld,
+                                                    int rows, int K, int rows_pad, int Kp,
+                                                    __bf16 *__restrict__ planes,
+                                                    int64_t plane_elems) {
+  __shared__ float tile[64][65];
+  const int k0 = blockIdx.x * 64, r0 = blockIdx.y * 64;
+  const int t = threadIdx.x;
+#pragma unroll
+  for (int s = 0; s < 16; ++s) {
+    int rl, kl;
+    if (R_CONTIG) {
+      rl = t & 63;
+      kl = (t >> 6) + 4 * s;
+    } else {
+      kl = t & 63;
+      rl = (t >> 6) + 4 * s;
+    }
+    const int r = r0 + rl, k = k0 + kl;
+    float v = 0.f;
+    if (r < rows && k < K) v = R_CONTIG ? x[r + (int64_t)k * ld] : x[k + (int64_t)r * ld];
+    tile[rl][kl] = v;
+  }
+  __syncthreads();
+  // Thread t writes row r0 + t%64, k-block k0/16 + t/64 (16 values, 32 B per plane).
+  const int rl = t & 63, kbl = t >> 6;
+  const int kb = k0 / 16 + kbl;
+  if (kb * 16 >= Kp) return;
+  bf16x8 h[2], m[2], l[2];
+#pragma unroll
+  for (int e = 0; e < 16; ++e) {
+    const float v = tile[rl][kbl * 16 + e];
+    const __bf16 hv = (__bf16)v;
+    const float r1 = v - (float)hv;
+    const __bf16 mv = (__bf16)r1;
+    h[e >> 3][e & 7] = hv;
+    m[e >> 3][e & 7] = mv;
+    if (PLANES == 3) l[e >> 3][e & 7] = (__bf16)(r1 - (float)mv);
+  }
+  const int64_t o = ((int64_t)kb * rows_pad + r0 + rl) * 16;
+  bf16x8 *p0 = (bf16x8 *)(planes + o);
+  bf16x8 *p1 = (bf16x8 *)(planes + plane_elems + o);
+  p0[0] = h[0];
+  p0[1] = h[1];
+  p1[0] = m[0];
+  p1[1] = m[1];
+  if (PLANES == 3) {
+    bf16x8 *p2 = (bf16x8 *)(planes + 2 * plane_elems + o);
+    p2[0] = l[0];
+    p2[1] = l[1];
+  }
+}
+
+// ----------------------------------------------------------------- gemm
+struct SplitArgs {
+  const __bf16 *a;  // planes of op(A): m_pad x Kp each, k-blocked by 16
+  const __bf16 *b;  // planes of op(B)^T: n_pad x Kp each, k-blocked by 16
+  int64_t a_plane, b_plane;  // elements per plane
+  int m, n, m_pad, n_pad, Kp;
+  float alpha, beta;
+  float *c;
+  int64_t ldc;
+};
+
+template <int N>
+__device__ __forceinline__ void stage_barrier() {
+  static_assert(N >= 0 && N < 64, "vmcnt range");
+  asm volatile("" ::: "memory");
+  __builtin_amdgcn_s_waitcnt((N & 0xf) | ((N >> 4) << 14) | 0x0070);  // vmcnt(N) lgkmcnt(0)
+  __builtin_amdgcn_s_barrier();
+  asm volatile("" ::: "memory");
+}
+
+__device__ __forceinline__ void tile_coords(int tiles_m, int tiles_n, int &tm, int &tn) {
+  constexpr int kXcds = 8;
+  constexpr int kGroupM = 8;
+  const int total = tiles_m * tiles_n;
+  int t = blockIdx.x;
+  if (total % kXcds == 0) t = (t % kXcds) * (total / kXcds) + t / kXcds;
+  const int per_group = kGroupM * tiles_n;
+  const int group = t / per_group;
+  const int first_m = group * kGroupM;
+  const int gm = min(tiles_m - first_m, kGroupM);
+  const int in_group = t % per_group;
+  tm = first_m + in_group % gm;
+  tn = in_group / gm;
+}
+
+// Buffer resource over one 1 KB block (wave-uniform base).
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t block_rsrc(const char *base) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<char *>(base), (short)0, 1024, 0x00020000);
+}
+
+template <int PL>
+struct SplitGemm {
+  static constexpr int kPerWave = 2 * PL;        // DMA blocks (1 KB = 32 rows) per operand per wave
+  static constexpr int kPerTile = 2 * kPerWave;  // DMA instructions per wave per k-tile
+  const char *ga[kPerWave];  // start of each of this wave's A blocks at k-block 0
+  const char *gb[kPerWave];
+  int64_t sa, sb;            // bytes from one k-block of a plane to the next
+  f32x16 acc[4][4];
+  int ktiles, lane, wave, wm, wn;
+
+  __device__ __forceinline__ void init(const SplitArgs &p, int m0, int n0) {
+    // Block j is 32 rows x 32 B, one contiguous kilobyte of a plane's k-block
+    // (wave-uniform address); lane l carries its bytes 16l .. 16l+15 (row
+    // l/2, k-chunk l%2).
+#pragma unroll
+    for (int j = 0; j < kPerWave; ++j) {
+      const int q = wave * kPerWave + j;  // block 0 .. 8*PL-1 (wave is uniform)
+      const int s = q >> 3, blk = q & 7;
+      ga[j] = (const char *)(p.a + s * p.a_plane + (int64_t)(m0 + blk * 32) * 16);
+      gb[j] = (const char *)(p.b + s * p.b_plane + (int64_t)(n0 + blk * 32) * 16);
+    }
+    sa = (int64_t)p.m_pad * 32;
+    sb = (int64_t)p.n_pad * 32;
+  }
+
+  // Block q = wave*kPerWave + j of a stage lands at byte q*1024 (plane q/8,
+  // rows 32*(q%8) ..). The LDS address is formed directly from the stage's
+  // own array so the compiler can see that filling stage S never touches the
+  // stage being read, and inserts no wait for it.
+  template <int S>
+  __device__ __forceinline__ void issue(int kt) {
+#pragma unroll
+    for (int j = 0; j < kPerWave; ++j)
+      __builtin_amdgcn_raw_ptr_buffer_load_lds(
+          block_rsrc(ga[j] + kt * sa), (lds_void_ptr)(lds_a<S>() + (wave * kPerWave + j) * 1024),
+          16, lane * 16, 0, 0, 0);
+#pragma unroll
+    for (int j = 0; j < kPerWave; ++j)
+      __builtin_amdgcn_raw_ptr_buffer_load_lds(
+          block_rsrc(gb[j] + kt * sb), (lds_void_ptr)(lds_b<S>() + (wave * kPerWave + j) * 1024),
+          16, lane * 16, 0, 0, 0);
+  }
+
+
+  template <int S>
+  __device__ __forceinline__ void compute() {
+    const char *As = lds_a<S>();
+    const char *Bs = lds_b<S>();
+    const int r32 = lane & 31, c = lane >> 5;
+    bf16x8 fa[PL][4], fb[PL][4];
+#pragma unroll
+    for (int s = 0; s < PL; ++s)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        fa[s][i] = *(const bf16x8 *)(As + lds_off(s, wm + 32 * i + r32, c));
+        fb[s][i] = *(const bf16x8 *)(Bs + lds_off(s, wn + 32 * i + r32, c));
+      }
+    // Smallest terms first.
+#pragma unroll
+    for (int pr = 0; pr < (PL == 2 ? 3 : 6); ++pr) {
+      int sa, sb;
+      if constexpr (PL == 2) {
+        sa = pr == 0 ? 1 : 0;
+        sb = pr == 1 ? 1 : 0;
+      } else {
+        constexpr int kA[6] = {2, 1, 0, 1, 0, 0};
+        constexpr int kB[6] = {0, 1, 2, 0, 1, 0};
+        sa = kA[pr];
+        sb = kB[pr];
+      }
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+          acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fb[sb][j], fa[sa][i], acc[i][j],
+                                                              0, 0, 0);
+    }
+  }
+
+  // Pipeline step for k-tile kt (stage S). The k extent is padded to a
+  // multiple of kStages tiles and the planes carry kStages - 1 spare k-blocks,
+  // so every step issues the tile kStages - 1 ahead unconditionally and
+  // exactly one tile (kPerTile instructions) is always in flight behind kt:
+  // the loop body has no branches for the compiler to merge across stages.
+  template <int S>
+  __device__ __forceinline__ void step(int kt) {
+    stage_barrier<kPerTile>();
+    issue<(S + kStages - 1) % kStages>(kt + kStages - 1);
+    compute<S>();
+  }
+};
+
+template <int PL>
+__global__ __launch_bounds__(kThreads) void split_gemm_kernel(SplitArgs p) {
+  SplitGemm<PL> g;
+  const int tiles_m = (p.m + kTile - 1) / kTile, tiles_n = (p.n + kTile - 1) / kTile;
+  int tm, tn;
+  tile_coords(tiles_m, tiles_n, tm, tn);
+  const int m0 = tm * kTile, n0 = tn * kTile;
+  const int tid = threadIdx.x;
+  g.wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  g.lane = tid & 63;
+  g.wm = (g.wave & 1) * kWave;
+  g.wn = (g.wave >> 1) * kWave;
+  g.ktiles = p.Kp / kBK;
+  g.init(p, m0, n0);
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) g.acc[i][j] = f32x16{};
+
+  g.template issue<0>(0);
+  g.template issue<1>(1);
+  for (int kt = 0; kt < g.ktiles; kt += kStages) {
+    g.template step<0>(kt);
+    g.template step<1>(kt + 1);
+    g.template step<2>(kt + 2);
+  }
+  stage_barrier<0>();  // the spare tiles' loads land before the workgroup exits
+
+  // Element r of lane l in block (i, j): C row m0+wm+32i+l%32, column
+  // n0+wn+32j+8(r/4)+4(l/32)+r%4.
+  const bool read_c = p.beta != 0.f;
+  const int r32 = g.lane & 31, hi = g.lane >> 5;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int gi = m0 + g.wm + 32 * i + r32;
+    if (gi >= p.m) continue;
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const int gj = n0 + g.wn + 32 * j + 8 * (r >> 2) + 4 * hi + (r & 3);
+        if (gj >= p.n) continue;
+        float *cp = p.c + gi + (int64_t)gj * p.ldc;
+        const float v = g.acc[i][j][r];
+        *cp = read_c ? __builtin_fmaf(p.alpha, v, p.beta * *cp) : p.alpha * v;
+      }
+  }
+}
+
+int64_t round_up(int64_t x, int64_t m) { return (x + m - 1) / m * m; }
+
+// k extent of the planes: a multiple of kStages k-tiles, plus kStages - 1
+// spare k-blocks that the pipeline prefetches past the end (never used).
+int64_t padded_k(int64_t k) { return round_up(k, (int64_t)kBK * kStages); }
+int64_t alloc_k(int64_t k) { return padded_k(k) + (int64_t)kBK * (kStages - 1); }
+
+template <int PLANES>
+void launch_split(bool r_contig, const float *x, int64_t ld, int rows, int K, int rows_pad, int Kp,
+                  __bf16 *planes, int64_t plane_elems, hipStream_t st) {
+  dim3 grid((unsigned)((Kp + 63) / 64), (unsigned)(rows_pad / 64));
+  if (r_contig)
+    hipLaunchKernelGGL((split_kernel<true, PLANES>), grid, dim3(256), 0, st, x, ld, rows, K,
+                       rows_pad, Kp, planes, plane_elems);
+  else
+    hipLaunchKernelGGL((split_kernel<false, PLANES>), grid, dim3(256), 0, st, x, ld, rows, K,
+                       rows_pad, Kp, planes, plane_elems);
+}
+
+}  // namespace
+
+size_t sgemm_split_workspace(int64_t m, int64_t n, int64_t k, int planes) {
+  const int64_t ka = alloc_k(k), mp = round_up(m, kTile), np = round_up(n, kTile);
+  return (size_t)planes * (size_t)(mp + np) * (size_t)ka * 2;
+}
+
+hipError_t launch_sgemm_split(const GemmProblem<float> &p, int planes, void *workspace,
+                              hipStream_t stream) {
+  if (planes != 2 && planes != 3) return hipErrorInvalidValue;
+  const int64_t kp = padded_k(p.k), ka = alloc_k(p.k);
+  const int64_t mp = round_up(p.m, kTile), np = round_up(p.n, kTile);
+  __bf16 *wa = (__bf16 *)workspace;
+  __bf16 *wb = wa + (int64_t)planes * mp * ka;
+  const int64_t pa = mp * ka, pb = np * ka;
+  // op(A) = A (m x k, m-contiguous) or A^T (stored k x m, k-contiguous);
+  // op(B)^T = B^T (B stored k x n, k-contiguous) or B (stored n x k, n-contiguous).
+  if (planes == 2) {
+    launch_split<2>(!p.ta, p.a, p.lda, (int)p.m, (int)p.k, (int)mp, (int)kp, wa, pa, stream);
+    launch_split<2>(p.tb, p.b, p.ldb, (int)p.n, (int)p.k, (int)np, (int)kp, wb, pb, stream);
+  } else {
+    launch_split<3>(!p.ta, p.a, p.lda, (int)p.m, (int)p.k, (int)mp, (int)kp, wa, pa, stream);
+    launch_split<3>(p.tb, p.b, p.ldb, (int)p.n, (int)p.k, (int)np, (int)kp, wb, pb, stream);
+  }
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  SplitArgs a{wa, wb, pa, pb, (int)p.m, (int)p.n, (int)mp, (int)np, (int)kp, p.alpha, p.beta,
+              p.c, p.ldc};
+  const int64_t tiles = (mp / kTile) * (np / kTile);
+  if (planes == 2)
+    hipLaunchKernelGGL((split_gemm_kernel<2>), dim3((unsigned)tiles), dim3(kThreads), 0, stream, a);
+  else
+    hipLaunchKernelGGL((split_gemm_kernel<3>), dim3((unsigned)tiles), dim3(kThreads), 0, stream, a);
+  return hipGetLastError();
+}
+
+}  // namespace rblas

@@ -36,4 +36,12 @@ template <typename T>
 hipError_t launch_scale(int64_t m, int64_t n, T beta, T *c, int64_t ldc,
                         hipStream_t stream);
 
+// SGEMM through bf16 splits (planes = 2: BF16x3, planes = 3: BF16x6). Same
+// preconditions as launch_gemm. `workspace` holds the split planes and must
+// have sgemm_split_workspace() bytes; it is free again once the work enqueued
+// on `stream` completes.
+size_t sgemm_split_workspace(int64_t m, int64_t n, int64_t k, int planes);
+hipError_t launch_sgemm_split(const GemmProblem<float> &p, int planes, void *workspace,
+                              hipStream_t stream);
+
 }  // namespace rblas

@@ -15,6 +15,29 @@ std::vector<Ctx *> g_pool;
 
 void set_error(const std::string &msg) { g_msg = msg; }
 
+hipError_t pool_alloc(void **ptr, size_t bytes, hipStream_t stream) {
+  static std::mutex mu;
+  static std::vector<int> configured;
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  {
+    std::lock_guard<std::mutex> g(mu);
+    bool done = false;
+    for (int d : configured) done |= d == dev;
+    if (!done) {
+      hipMemPool_t pool;
+      e = hipDeviceGetDefaultMemPool(&pool, dev);
+      if (e != hipSuccess) return e;
+      uint64_t keep = ~0ull;
+      e = hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
+      if (e != hipSuccess) return e;
+      configured.push_back(dev);
+    }
+  }
+  return hipMallocAsync(ptr, bytes, stream);
+}
+
 int32_t fail(const Check &c, rblas_error_t *err) {
   if (err) *err = c.err;
   if (c.status != RBLAS_OK) set_error(c.msg);

@@ -13,6 +13,11 @@
 namespace rblas {
 
 void set_error(const std::string &msg);
+
+// Stream-ordered device allocation from the current device's default memory
+// pool, which is configured once to keep freed memory cached (no release to
+// the OS between calls). Free with hipFreeAsync on the same stream.
+hipError_t pool_alloc(void **ptr, size_t bytes, hipStream_t stream);
 int32_t fail(const Check &c, rblas_error_t *err);
 int32_t hip_fail(hipError_t e, const char *what);
 

@@ -96,3 +96,27 @@ def gemm_dev(layout, transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ld
             a.ptr, a.size, lda, b.ptr, b.size, ldb, _scalar(pre, beta), c.ptr, c.size, ldc,
             stream.handle if stream is not None else None)
     raise_status(st)
+
+
+def sgemm_ex(layout, transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc,
+             compute) -> None:
+    """f32 gemm with a chosen arithmetic (types.Compute); host slices."""
+    if _prefix(a, b, c) != "s":
+        raise TypeError("sgemm_ex takes float32 operands")
+    st = _ffi.lib().rblas_sgemm_ex(int(layout), int(transa), int(transb), m, n, k,
+                                   C.c_float(alpha), a.ctypes.data, a.size, lda, b.ctypes.data,
+                                   b.size, ldb, C.c_float(beta), c.ctypes.data, c.size, ldc,
+                                   int(compute))
+    raise_status(st)
+
+
+def sgemm_ex_dev(layout, transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc,
+                 compute, stream=None) -> None:
+    """sgemm_ex on HBM-resident operands, enqueued on `stream`."""
+    if _dev_prefix(a, b, c) != "s":
+        raise TypeError("sgemm_ex_dev takes float32 operands")
+    st = _ffi.lib().rblas_sgemm_ex_dev(int(layout), int(transa), int(transb), m, n, k,
+                                       C.c_float(alpha), a.ptr, a.size, lda, b.ptr, b.size, ldb,
+                                       C.c_float(beta), c.ptr, c.size, ldc, int(compute),
+                                       stream.handle if stream is not None else None)
+    raise_status(st)

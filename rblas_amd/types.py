@@ -38,6 +38,14 @@ class Diag(enum.IntEnum):
     Unit = 1
 
 
+class Compute(enum.IntEnum):
+    """SGEMM arithmetic (rblas_sgemm_ex): plain f32, or f32 operands split
+    into 2 (BF16X3, 3 products) or 3 (BF16X6, 6 products) bf16 terms."""
+    F32 = 0
+    BF16X3 = 1
+    BF16X6 = 2
+
+
 # C-ABI status codes (include/rblas_hip.h)
 OK = 0
 INVALID_LEADING_DIM = 1
