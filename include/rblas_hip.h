@@ -149,6 +149,44 @@ int32_t rblas_sgemm_ex_dev(int32_t layout, int32_t transa, int32_t transb,
                            float beta, float *c, size_t c_len, size_t ldc,
                            int32_t compute, void *stream);
 
+/* ----------------------------------------------------------------- level-1
+ * Replace `level1::dot` (level1.rs:45), `level1::nrm2` (level1.rs:96) and
+ * `level1::axpy` (level1.rs:17). Strides are rblas's `usize` increments.
+ * RBLAS_PANIC where rblas panics: "slice shorter than n" on the contiguous
+ * path (inc == 1 for every vector), "stride must be positive" and "x/y out of
+ * bounds" on the strided path; axpy with n == 0 or alpha == 0 and dot/nrm2
+ * with n == 0 return before any check, as in rblas. dot/nrm2 write the
+ * result to `*result` (host memory; device memory for the `_dev` forms). */
+int32_t rblas_sdot(size_t n, const float *x, size_t x_len, size_t incx,
+                   const float *y, size_t y_len, size_t incy, float *result);
+int32_t rblas_ddot(size_t n, const double *x, size_t x_len, size_t incx,
+                   const double *y, size_t y_len, size_t incy, double *result);
+int32_t rblas_snrm2(size_t n, const float *x, size_t x_len, size_t incx,
+                    float *result);
+int32_t rblas_dnrm2(size_t n, const double *x, size_t x_len, size_t incx,
+                    double *result);
+int32_t rblas_saxpy(size_t n, float alpha, const float *x, size_t x_len,
+                    size_t incx, float *y, size_t y_len, size_t incy);
+int32_t rblas_daxpy(size_t n, double alpha, const double *x, size_t x_len,
+                    size_t incx, double *y, size_t y_len, size_t incy);
+/* HBM-resident forms (no rblas counterpart), enqueued on `stream`. */
+int32_t rblas_sdot_dev(size_t n, const float *x, size_t x_len, size_t incx,
+                       const float *y, size_t y_len, size_t incy,
+                       float *result, void *stream);
+int32_t rblas_ddot_dev(size_t n, const double *x, size_t x_len, size_t incx,
+                       const double *y, size_t y_len, size_t incy,
+                       double *result, void *stream);
+int32_t rblas_snrm2_dev(size_t n, const float *x, size_t x_len, size_t incx,
+                        float *result, void *stream);
+int32_t rblas_dnrm2_dev(size_t n, const double *x, size_t x_len, size_t incx,
+                        double *result, void *stream);
+int32_t rblas_saxpy_dev(size_t n, float alpha, const float *x, size_t x_len,
+                        size_t incx, float *y, size_t y_len, size_t incy,
+                        void *stream);
+int32_t rblas_daxpy_dev(size_t n, double alpha, const double *x, size_t x_len,
+                        size_t incx, double *y, size_t y_len, size_t incy,
+                        void *stream);
+
 /* ------------------------------------------------- device memory & streams
  * Plumbing for callers that keep operands resident in HBM (no rblas
  * counterpart). All return a status. */

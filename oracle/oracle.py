@@ -185,3 +185,32 @@ def gemm_panics(layout, ta, tb, m, n, k, a_len, lda, b_len, ldb, c_len, ldc, alp
         return False
     touches_c = beta != 1 or (k != 0 and alpha != 0)
     return touches_c and (m - 1) + (n - 1) * ldc >= c_len
+
+
+def axpy_panics(n, alpha, x_len, incx, y_len, incy):
+    """level1.rs:17-41: no-op (no checks) when n == 0 or alpha == 0."""
+    if n == 0 or alpha == 0:
+        return False
+    if incx == 1 and incy == 1:
+        return not (x_len >= n and y_len >= n)
+    if not (incx > 0 and incy > 0):
+        return True
+    return not ((n - 1) * incx < x_len and (n - 1) * incy < y_len)
+
+
+def dot_panics(n, x_len, incx, y_len, incy):
+    """level1.rs:45-66."""
+    if n == 0:
+        return False
+    if incx == 1 and incy == 1:
+        return not (x_len >= n and y_len >= n)
+    if not (incx > 0 and incy > 0):
+        return True
+    return not ((n - 1) * incx < x_len and (n - 1) * incy < y_len)
+
+
+def nrm2_panics(n, x_len, incx):
+    """level1.rs:96-101."""
+    if n == 0:
+        return False
+    return not (incx > 0) or not ((n - 1) * incx < x_len)

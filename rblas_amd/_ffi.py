@@ -45,6 +45,18 @@ SIGNATURES: dict[str, tuple[list, object]] = {
     "rblas_sgemm_ex": (_gemm_args(f32) + [i32], i32),
     "rblas_sgemm_ex_dev": (_gemm_args(f32) + [i32, vp], i32),
     "rblas_dgemm_dev": (_gemm_args(f64) + [vp], i32),
+    "rblas_sdot": ([sz, vp, sz, sz, vp, sz, sz, C.POINTER(f32)], i32),
+    "rblas_ddot": ([sz, vp, sz, sz, vp, sz, sz, C.POINTER(f64)], i32),
+    "rblas_snrm2": ([sz, vp, sz, sz, C.POINTER(f32)], i32),
+    "rblas_dnrm2": ([sz, vp, sz, sz, C.POINTER(f64)], i32),
+    "rblas_saxpy": ([sz, f32, vp, sz, sz, vp, sz, sz], i32),
+    "rblas_daxpy": ([sz, f64, vp, sz, sz, vp, sz, sz], i32),
+    "rblas_sdot_dev": ([sz, vp, sz, sz, vp, sz, sz, vp, vp], i32),
+    "rblas_ddot_dev": ([sz, vp, sz, sz, vp, sz, sz, vp, vp], i32),
+    "rblas_snrm2_dev": ([sz, vp, sz, sz, vp, vp], i32),
+    "rblas_dnrm2_dev": ([sz, vp, sz, sz, vp, vp], i32),
+    "rblas_saxpy_dev": ([sz, f32, vp, sz, sz, vp, sz, sz, vp], i32),
+    "rblas_daxpy_dev": ([sz, f64, vp, sz, sz, vp, sz, sz, vp], i32),
     "rblas_malloc": ([C.POINTER(vp), sz], i32),
     "rblas_free": ([vp], i32),
     "rblas_memcpy_htod": ([vp, vp, sz, vp], i32),

@@ -61,5 +61,10 @@ def build(verbose: bool = False) -> str:
 
 
 if __name__ == "__main__":
-    build(verbose=True)
+    try:
+        build(verbose=True)
+    except RuntimeError as e:
+        print(e, file=sys.stderr)
+        print("BUILD FAILED", file=sys.stderr)
+        sys.exit(1)
     sys.exit(0)

@@ -163,6 +163,9 @@ __device__ __forceinline__ void tile_coords(int tiles_m, int tiles_n, int &tm, i
   tn = in_group / gm;
 }
 
+// LDS address-space view of a pointer into one of the stage arrays.
+__device__ __forceinline__ lds_void_ptr to_lds(char *p) { return (lds_void_ptr)p; }
+
 // Buffer resource over one 1 KB block (wave-uniform base).
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t block_rsrc(const char *base) {
   return __builtin_amdgcn_make_buffer_rsrc(const_cast<char *>(base), (short)0, 1024, 0x00020000);
@@ -197,18 +200,19 @@ struct SplitGemm {
   // rows 32*(q%8) ..). The LDS address is formed directly from the stage's
   // own array so the compiler can see that filling stage S never touches the
   // stage being read, and inserts no wait for it.
-  template <int S>
-  __device__ __forceinline__ void issue(int kt) {
+  // (A plain member taking the stage's arrays: a device builtin inside a
+  // member template fails clang's host-side pass.)
+  __device__ __forceinline__ void issue(int kt, char *la, char *lb) {
 #pragma unroll
     for (int j = 0; j < kPerWave; ++j)
-      __builtin_amdgcn_raw_ptr_buffer_load_lds(
-          block_rsrc(ga[j] + kt * sa), (lds_void_ptr)(lds_a<S>() + (wave * kPerWave + j) * 1024),
-          16, lane * 16, 0, 0, 0);
+      __builtin_amdgcn_raw_ptr_buffer_load_lds(block_rsrc(ga[j] + kt * sa),
+                                               to_lds(la + (wave * kPerWave + j) * 1024), 16,
+                                               lane * 16, 0, 0, 0);
 #pragma unroll
     for (int j = 0; j < kPerWave; ++j)
-      __builtin_amdgcn_raw_ptr_buffer_load_lds(
-          block_rsrc(gb[j] + kt * sb), (lds_void_ptr)(lds_b<S>() + (wave * kPerWave + j) * 1024),
-          16, lane * 16, 0, 0, 0);
+      __builtin_amdgcn_raw_ptr_buffer_load_lds(block_rsrc(gb[j] + kt * sb),
+                                               to_lds(lb + (wave * kPerWave + j) * 1024), 16,
+                                               lane * 16, 0, 0, 0);
   }
 
 
@@ -255,7 +259,8 @@ struct SplitGemm {
   template <int S>
   __device__ __forceinline__ void step(int kt) {
     stage_barrier<kPerTile>();
-    issue<(S + kStages - 1) % kStages>(kt + kStages - 1);
+    constexpr int N = (S + kStages - 1) % kStages;
+    issue(kt + kStages - 1, lds_a<N>(), lds_b<N>());
     compute<S>();
   }
 };
@@ -279,8 +284,8 @@ __global__ __launch_bounds__(kThreads) void split_gemm_kernel(SplitArgs p) {
 #pragma unroll
     for (int j = 0; j < 4; ++j) g.acc[i][j] = f32x16{};
 
-  g.template issue<0>(0);
-  g.template issue<1>(1);
+  g.issue(0, lds_a<0>(), lds_b<0>());
+  g.issue(1, lds_a<1>(), lds_b<1>());
   for (int kt = 0; kt < g.ktiles; kt += kStages) {
     g.template step<0>(kt);
     g.template step<1>(kt + 1);
@@ -289,7 +294,8 @@ __global__ __launch_bounds__(kThreads) void split_gemm_kernel(SplitArgs p) {
   stage_barrier<0>();  // the spare tiles' loads land before the workgroup exits
 
   // Element r of lane l in block (i, j): C row m0+wm+32i+l%32, column
-  // n0+wn+32j+8(r/4)+4(l/32)+r%4.
+  // n0+wn+32j+8(r/4)+4(l/32)+r%4. Each block's C values are loaded together
+  // before any of them is stored (one memory round trip per block).
   const bool read_c = p.beta != 0.f;
   const int r32 = g.lane & 31, hi = g.lane >> 5;
 #pragma unroll
@@ -297,15 +303,22 @@ __global__ __launch_bounds__(kThreads) void split_gemm_kernel(SplitArgs p) {
     const int gi = m0 + g.wm + 32 * i + r32;
     if (gi >= p.m) continue;
 #pragma unroll
-    for (int j = 0; j < 4; ++j)
+    for (int j = 0; j < 4; ++j) {
+      float cv[16];
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const int gj = n0 + g.wn + 32 * j + 8 * (r >> 2) + 4 * hi + (r & 3);
+        cv[r] = (read_c && gj < p.n) ? p.c[gi + (int64_t)gj * p.ldc] : 0.f;
+      }
 #pragma unroll
       for (int r = 0; r < 16; ++r) {
         const int gj = n0 + g.wn + 32 * j + 8 * (r >> 2) + 4 * hi + (r & 3);
         if (gj >= p.n) continue;
-        float *cp = p.c + gi + (int64_t)gj * p.ldc;
         const float v = g.acc[i][j][r];
-        *cp = read_c ? __builtin_fmaf(p.alpha, v, p.beta * *cp) : p.alpha * v;
+        p.c[gi + (int64_t)gj * p.ldc] =
+            read_c ? __builtin_fmaf(p.alpha, v, p.beta * cv[r]) : p.alpha * v;
       }
+    }
   }
 }
 

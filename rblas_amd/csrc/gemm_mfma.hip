@@ -413,7 +413,8 @@ __global__ __launch_bounds__(kThreads) void gemm_tile_kernel(GemmProblem<T> p) {
 
   // Epilogue: element (i, j) of C gets alpha*acc + beta*C; beta == 0 never
   // reads C. Lane l of block (fi, fj) owns row i = m0+wm+16fi+l%16 and
-  // columns j = n0+wn+16fj+row(l, r).
+  // columns j = n0+wn+16fj+row(l, r). Each block's C values are loaded
+  // together before any of them is stored.
   const T alpha = p.alpha, beta = p.beta;
   const bool read_c = beta != T(0);
   const int c16 = tg.lane & 15;
@@ -423,13 +424,18 @@ __global__ __launch_bounds__(kThreads) void gemm_tile_kernel(GemmProblem<T> p) {
     if (gi >= M) continue;
 #pragma unroll
     for (int j = 0; j < kFrag; ++j) {
+      T cv[4];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int gj = n0 + tg.wn + 16 * j + Mfma<T>::row(tg.lane, r);
+        cv[r] = (read_c && gj < N) ? p.c[gi + (int64_t)gj * p.ldc] : T(0);
+      }
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int gj = n0 + tg.wn + 16 * j + Mfma<T>::row(tg.lane, r);
         if (gj >= N) continue;
-        T *cp = p.c + gi + (int64_t)gj * p.ldc;
         const T v = tg.acc[i][j][r];
-        *cp = read_c ? __builtin_fma(alpha, v, beta * *cp) : alpha * v;
+        p.c[gi + (int64_t)gj * p.ldc] = read_c ? __builtin_fma(alpha, v, beta * cv[r]) : alpha * v;
       }
     }
   }

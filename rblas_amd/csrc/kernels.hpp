@@ -44,4 +44,17 @@ size_t sgemm_split_workspace(int64_t m, int64_t n, int64_t k, int planes);
 hipError_t launch_sgemm_split(const GemmProblem<float> &p, int planes, void *workspace,
                               hipStream_t stream);
 
+// Level-1 (HBM-bound). `scratch` holds per-block partials, at least
+// level1_scratch_bytes(); the result is written to device memory `out`.
+size_t level1_scratch_bytes();
+template <typename T>
+hipError_t launch_dot(int64_t n, const T *x, int64_t incx, const T *y, int64_t incy, T *out,
+                      void *scratch, hipStream_t stream);
+template <typename T>
+hipError_t launch_nrm2(int64_t n, const T *x, int64_t incx, T *out, void *scratch,
+                       hipStream_t stream);
+template <typename T>
+hipError_t launch_axpy(int64_t n, T alpha, const T *x, int64_t incx, T *y, int64_t incy,
+                       hipStream_t stream);
+
 }  // namespace rblas
