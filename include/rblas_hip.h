@@ -149,6 +149,94 @@ int32_t rblas_sgemm_ex_dev(int32_t layout, int32_t transa, int32_t transb,
                            float beta, float *c, size_t c_len, size_t ldc,
                            int32_t compute, void *stream);
 
+/* ------------------------------------------------------------ complex gemm
+ * C := alpha * op(A) * op(B) + beta * C on interleaved complex matrices,
+ * column-major (rblas's complex gemm has no Layout argument). Replaces
+ * `complex3::gemm` (complex3.rs:27-74); op codes as Transpose, ConjTrans
+ * conjugating. Lengths and leading dimensions count complex elements.
+ * RBLAS_PANIC where rblas indexes past a slice (split_op's gathers of A and
+ * B, the final update of C). Computed by the 4M decomposition on the device
+ * (two real MFMA gemms with k concatenated, then the complex alpha/beta
+ * update); beta == 0 does not read C, as in rblas. */
+int32_t rblas_cgemm(int32_t transa, int32_t transb, size_t m, size_t n, size_t k,
+                    rblas_complex32_t alpha, const rblas_complex32_t *a,
+                    size_t a_len, size_t lda, const rblas_complex32_t *b,
+                    size_t b_len, size_t ldb, rblas_complex32_t beta,
+                    rblas_complex32_t *c, size_t c_len, size_t ldc);
+int32_t rblas_zgemm(int32_t transa, int32_t transb, size_t m, size_t n, size_t k,
+                    rblas_complex64_t alpha, const rblas_complex64_t *a,
+                    size_t a_len, size_t lda, const rblas_complex64_t *b,
+                    size_t b_len, size_t ldb, rblas_complex64_t beta,
+                    rblas_complex64_t *c, size_t c_len, size_t ldc);
+/* HBM-resident forms (no rblas counterpart); the 4M planes come from the
+ * stream-ordered pool. */
+int32_t rblas_cgemm_dev(int32_t transa, int32_t transb, size_t m, size_t n,
+                        size_t k, rblas_complex32_t alpha,
+                        const rblas_complex32_t *a, size_t a_len, size_t lda,
+                        const rblas_complex32_t *b, size_t b_len, size_t ldb,
+                        rblas_complex32_t beta, rblas_complex32_t *c,
+                        size_t c_len, size_t ldc, void *stream);
+int32_t rblas_zgemm_dev(int32_t transa, int32_t transb, size_t m, size_t n,
+                        size_t k, rblas_complex64_t alpha,
+                        const rblas_complex64_t *a, size_t a_len, size_t lda,
+                        const rblas_complex64_t *b, size_t b_len, size_t ldb,
+                        rblas_complex64_t beta, rblas_complex64_t *c,
+                        size_t c_len, size_t ldc, void *stream);
+
+/* -------------------------------------------------------------------- syrk
+ * C := alpha * op(A) * op(A)^T + beta * C on the `uplo` triangle of the
+ * symmetric n x n C, column-major; op(A) is n x k (stored n x k for
+ * Transpose None, k x n otherwise). Replaces `level3::syrk`
+ * (level3.rs:236-294). n == 0 returns at once. RBLAS_PANIC where rblas
+ * indexes past a slice: the gathers of op(A) of its inner gemm, then the
+ * triangle's largest C index (n-1)*(ldc+1). beta == 0 stores the triangle
+ * without reading it. Only the tiles on or inside the triangle are computed.
+ * RBLAS_BAD_ARG for ldc < n with n > 1 (overlapping C columns). */
+int32_t rblas_ssyrk(int32_t uplo, int32_t trans, size_t n, size_t k, float alpha,
+                    const float *a, size_t a_len, size_t lda, float beta,
+                    float *c, size_t c_len, size_t ldc);
+int32_t rblas_dsyrk(int32_t uplo, int32_t trans, size_t n, size_t k,
+                    double alpha, const double *a, size_t a_len, size_t lda,
+                    double beta, double *c, size_t c_len, size_t ldc);
+/* HBM-resident forms (no rblas counterpart). */
+int32_t rblas_ssyrk_dev(int32_t uplo, int32_t trans, size_t n, size_t k,
+                        float alpha, const float *a, size_t a_len, size_t lda,
+                        float beta, float *c, size_t c_len, size_t ldc,
+                        void *stream);
+int32_t rblas_dsyrk_dev(int32_t uplo, int32_t trans, size_t n, size_t k,
+                        double alpha, const double *a, size_t a_len, size_t lda,
+                        double beta, double *c, size_t c_len, size_t ldc,
+                        void *stream);
+
+/* -------------------------------------------------------------------- trsm
+ * Solves op(A) X = alpha B (side Left, A m x m) or X op(A) = alpha B (Right,
+ * A n x n) in place in B (m x n), column-major, A triangular (`uplo`, `diag`
+ * Unit: the stored diagonal is not read for the solve). Replaces
+ * `level3::trsm` (level3.rs:390-520). m == 0 or n == 0 returns at once.
+ * RBLAS_PANIC where rblas indexes past a slice: B's largest index
+ * (m-1) + (n-1)*ldb, A's diagonal (q-1)*(lda+1). No singularity check (a zero
+ * diagonal gives Inf/NaN as in rblas). Blocked on the device: the 64 x 64
+ * diagonal blocks of op(A) are inverted in FP64, the rest is MFMA gemm.
+ * RBLAS_BAD_ARG for ldb < m with n > 1 (overlapping B columns). */
+int32_t rblas_strsm(int32_t side, int32_t uplo, int32_t trans, int32_t diag,
+                    size_t m, size_t n, float alpha, const float *a,
+                    size_t a_len, size_t lda, float *b, size_t b_len,
+                    size_t ldb);
+int32_t rblas_dtrsm(int32_t side, int32_t uplo, int32_t trans, int32_t diag,
+                    size_t m, size_t n, double alpha, const double *a,
+                    size_t a_len, size_t lda, double *b, size_t b_len,
+                    size_t ldb);
+/* HBM-resident forms (no rblas counterpart); the workspace comes from the
+ * stream-ordered pool. */
+int32_t rblas_strsm_dev(int32_t side, int32_t uplo, int32_t trans, int32_t diag,
+                        size_t m, size_t n, float alpha, const float *a,
+                        size_t a_len, size_t lda, float *b, size_t b_len,
+                        size_t ldb, void *stream);
+int32_t rblas_dtrsm_dev(int32_t side, int32_t uplo, int32_t trans, int32_t diag,
+                        size_t m, size_t n, double alpha, const double *a,
+                        size_t a_len, size_t lda, double *b, size_t b_len,
+                        size_t ldb, void *stream);
+
 /* ----------------------------------------------------------------- level-1
  * Replace `level1::dot` (level1.rs:45), `level1::nrm2` (level1.rs:96) and
  * `level1::axpy` (level1.rs:17). Strides are rblas's `usize` increments.

@@ -214,3 +214,73 @@ def nrm2_panics(n, x_len, incx):
     if n == 0:
         return False
     return not (incx > 0) or not ((n - 1) * incx < x_len)
+
+
+def syrk_panics(uplo, trans, n, k, a_len, lda, c_len, ldc):
+    """level3.rs:236-294 walked literally (small sizes): n == 0 returns; the
+    inner gemm is never NN, so materialize gathers op(A) (n x k) and op(A)^T
+    (k x n) from `a` (level3.rs:193-220); then the fold indexes c[i + j*ldc]
+    over the uplo triangle. True if any index reaches past its slice."""
+    if n == 0:
+        return False
+    t = trans != NO_TRANS
+    for i in range(n):          # op(A)(i, p)
+        for p in range(k):
+            if (p + i * lda if t else i + p * lda) >= a_len:
+                return True
+    for p in range(k):          # op(A)^T(p, j) = op(A)(j, p)
+        for j in range(n):
+            if (p + j * lda if t else j + p * lda) >= a_len:
+                return True
+    for j in range(n):
+        lo, hi = (0, j + 1) if uplo == 0 else (j, n)
+        for i in range(lo, hi):
+            if i + j * ldc >= c_len:
+                return True
+    return False
+
+
+def trsm_panics(side, uplo, trans, diag, m, n, alpha, a_len, lda, b_len, ldb):
+    """level3.rs:390-520 walked literally (small sizes): the alpha pre-scale,
+    then solve_left per column or solve_right per row, recording every a/b
+    index (finish's a_ii argument is evaluated for Unit too). True if any
+    index reaches past its slice."""
+    t = trans != NO_TRANS
+    idx_a = (lambda r, c: c + r * lda) if t else (lambda r, c: r + c * lda)
+    eff_upper = (uplo == 0) != t
+    bad = [False]
+
+    def ra(r, c):
+        if idx_a(r, c) >= a_len:
+            bad[0] = True
+
+    def rb(i):
+        if i >= b_len:
+            bad[0] = True
+
+    if alpha != 1:
+        for j in range(n):
+            for i in range(m):
+                rb(i + j * ldb)
+    if side == 0:
+        for j in range(n):
+            col = j * ldb
+            rows = range(m - 1, -1, -1) if eff_upper else range(m)
+            for i in rows:
+                rb(i + col)
+                ks = range(i + 1, m) if eff_upper else range(i)
+                for kk in ks:
+                    ra(i, kk)
+                    rb(kk + col)
+                ra(i, i)
+    else:
+        order = range(n) if eff_upper else range(n - 1, -1, -1)
+        for i in range(m):
+            for j in order:
+                rb(i + j * ldb)
+                ks = range(j) if eff_upper else range(j + 1, n)
+                for kk in ks:
+                    rb(i + kk * ldb)
+                    ra(kk, j)
+                ra(j, j)
+    return bad[0]

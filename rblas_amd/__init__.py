@@ -5,5 +5,5 @@ Python face of librblas_hip.so, mirroring rblas's modules:
   device   HBM buffers, streams and events for resident operands
   types    Layout, Transpose, Uplo, Side, Diag, BlasError
 """
-from . import level1, level3, types  # noqa: F401
+from . import complex3, level1, level3, types  # noqa: F401
 from .types import BlasError, BlasPanic, Diag, Layout, Side, Transpose, Uplo  # noqa: F401

@@ -45,6 +45,22 @@ SIGNATURES: dict[str, tuple[list, object]] = {
     "rblas_sgemm_ex": (_gemm_args(f32) + [i32], i32),
     "rblas_sgemm_ex_dev": (_gemm_args(f32) + [i32, vp], i32),
     "rblas_dgemm_dev": (_gemm_args(f64) + [vp], i32),
+    "rblas_cgemm": ([i32, i32, sz, sz, sz, Complex32, vp, sz, sz, vp, sz, sz, Complex32, vp, sz,
+                     sz], i32),
+    "rblas_zgemm": ([i32, i32, sz, sz, sz, Complex64, vp, sz, sz, vp, sz, sz, Complex64, vp, sz,
+                     sz], i32),
+    "rblas_cgemm_dev": ([i32, i32, sz, sz, sz, Complex32, vp, sz, sz, vp, sz, sz, Complex32, vp,
+                         sz, sz, vp], i32),
+    "rblas_zgemm_dev": ([i32, i32, sz, sz, sz, Complex64, vp, sz, sz, vp, sz, sz, Complex64, vp,
+                         sz, sz, vp], i32),
+    "rblas_ssyrk": ([i32, i32, sz, sz, f32, vp, sz, sz, f32, vp, sz, sz], i32),
+    "rblas_dsyrk": ([i32, i32, sz, sz, f64, vp, sz, sz, f64, vp, sz, sz], i32),
+    "rblas_ssyrk_dev": ([i32, i32, sz, sz, f32, vp, sz, sz, f32, vp, sz, sz, vp], i32),
+    "rblas_dsyrk_dev": ([i32, i32, sz, sz, f64, vp, sz, sz, f64, vp, sz, sz, vp], i32),
+    "rblas_strsm": ([i32, i32, i32, i32, sz, sz, f32, vp, sz, sz, vp, sz, sz], i32),
+    "rblas_dtrsm": ([i32, i32, i32, i32, sz, sz, f64, vp, sz, sz, vp, sz, sz], i32),
+    "rblas_strsm_dev": ([i32, i32, i32, i32, sz, sz, f32, vp, sz, sz, vp, sz, sz, vp], i32),
+    "rblas_dtrsm_dev": ([i32, i32, i32, i32, sz, sz, f64, vp, sz, sz, vp, sz, sz, vp], i32),
     "rblas_sdot": ([sz, vp, sz, sz, vp, sz, sz, C.POINTER(f32)], i32),
     "rblas_ddot": ([sz, vp, sz, sz, vp, sz, sz, C.POINTER(f64)], i32),
     "rblas_snrm2": ([sz, vp, sz, sz, C.POINTER(f32)], i32),

@@ -1,0 +1,47 @@
+"""Complex Level-3 on the GPU with rblas's signature (complex3.rs).
+
+a, b, c are 1-D numpy complex64 / complex128 arrays (interleaved {re, im},
+rblas's Complex<T> layout), column-major; c is updated in place.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from . import _ffi
+from .level3 import raise_status
+
+
+def _kind(*arrs):
+    dt = arrs[-1].dtype
+    for x in arrs:
+        if x.dtype != dt:
+            raise TypeError("all operands must share one dtype")
+        if x.ndim != 1 or not x.flags.c_contiguous:
+            raise TypeError("operands are 1-D contiguous arrays (rblas slices)")
+    if dt == np.complex64:
+        return "c", _ffi.Complex32
+    if dt == np.complex128:
+        return "z", _ffi.Complex64
+    raise TypeError(f"unsupported dtype {dt}")
+
+
+def gemm(transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc) -> None:
+    """C := alpha*op(A)*op(B) + beta*C by 4M (complex3.rs:27-74)."""
+    pre, cs = _kind(a, b, c)
+    al, be = complex(alpha), complex(beta)
+    st = getattr(_ffi.lib(), f"rblas_{pre}gemm")(
+        int(transa), int(transb), m, n, k, cs(al.real, al.imag), a.ctypes.data, a.size, lda,
+        b.ctypes.data, b.size, ldb, cs(be.real, be.imag), c.ctypes.data, c.size, ldc)
+    raise_status(st)
+
+
+def gemm_dev(transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, stream=None) -> None:
+    """gemm on HBM-resident complex operands (device.DeviceArray)."""
+    pre = {np.dtype(np.complex64): "c", np.dtype(np.complex128): "z"}[c.dtype]
+    cs = _ffi.Complex32 if pre == "c" else _ffi.Complex64
+    al, be = complex(alpha), complex(beta)
+    st = getattr(_ffi.lib(), f"rblas_{pre}gemm_dev")(
+        int(transa), int(transb), m, n, k, cs(al.real, al.imag), a.ptr, a.size, lda, b.ptr,
+        b.size, ldb, cs(be.real, be.imag), c.ptr, c.size, ldc,
+        stream.handle if stream is not None else None)
+    raise_status(st)

@@ -1,0 +1,165 @@
+// C-ABI complex gemm: complex3::gemm (complex3.rs:27-74), column-major,
+// interleaved Complex<T>, host slices and HBM-resident forms.
+#include <climits>
+
+#include "kernels.hpp"
+#include "runtime.hpp"
+
+namespace rblas {
+namespace {
+
+// Where complex3::gemm panics: split_op's gathers of op(A) (m x k) and op(B)
+// (k x n) index past their slices (complex3.rs:111-118), or the final
+// combine indexes C past its slice (complex3.rs:61-73). Lengths are in
+// complex elements.
+Check cgemm_panic_check(bool at, bool bt, size_t m, size_t n, size_t k, size_t a_len, size_t lda,
+                        size_t b_len, size_t ldb, size_t c_len, size_t ldc) {
+  if (m > 0 && k > 0) {
+    size_t hi;
+    bool fits = at ? max_index(m, k, lda, 1, hi) : max_index(m, k, 1, lda, hi);
+    if (!fits || hi >= a_len)
+      return panic("index out of bounds: the len is " + std::to_string(a_len) +
+                   " but the index is " + (fits ? std::to_string(hi) : std::string("past usize")));
+  }
+  if (k > 0 && n > 0) {
+    size_t hi;
+    bool fits = bt ? max_index(k, n, ldb, 1, hi) : max_index(k, n, 1, ldb, hi);
+    if (!fits || hi >= b_len)
+      return panic("index out of bounds: the len is " + std::to_string(b_len) +
+                   " but the index is " + (fits ? std::to_string(hi) : std::string("past usize")));
+  }
+  if (m > 0 && n > 0) {
+    size_t hi;
+    bool fits = max_index(m, n, 1, ldc, hi);
+    if (!fits || hi >= c_len)
+      return panic("index out of bounds: the len is " + std::to_string(c_len) +
+                   " but the index is " + (fits ? std::to_string(hi) : std::string("past usize")));
+  }
+  return Check{};
+}
+
+template <typename T>
+Check cgemm_validate(int32_t ta, int32_t tb, size_t m, size_t n, size_t k, size_t a_len,
+                     size_t lda, size_t b_len, size_t ldb, size_t c_len, size_t ldc) {
+  if (!valid_trans(ta) || !valid_trans(tb)) {
+    Check bad;
+    bad.status = RBLAS_BAD_ENUM;
+    bad.err = {RBLAS_BAD_ENUM, 0, 0, 0};
+    bad.msg = "complex gemm: enum argument out of range";
+    return bad;
+  }
+  Check c = cgemm_panic_check(is_transposed(ta), is_transposed(tb), m, n, k, a_len, lda, b_len,
+                              ldb, c_len, ldc);
+  if (!c.ok()) return c;
+  // The real gemms run with k' = 2k and leading dimensions m and 2k.
+  if (m > INT_MAX || n > INT_MAX || 2 * k > INT_MAX ||
+      ((m > 2 * k ? m : 2 * k) * 128 + 160) * sizeof(T) >= 0x7fffffe0ull) {
+    Check bad;
+    bad.status = RBLAS_BAD_ARG;
+    bad.err = {RBLAS_BAD_ARG, 0, 0, 0};
+    bad.msg = "complex gemm: dimensions too large for the device path";
+    return bad;
+  }
+  return Check{};
+}
+
+size_t cext(size_t rows, size_t cols, size_t ld) {
+  return rows == 0 || cols == 0 ? 0 : (cols - 1) * ld + rows;
+}
+
+template <typename T>
+int32_t cgemm_host(int32_t ta, int32_t tb, size_t m, size_t n, size_t k, T ar, T ai, const T *a,
+                   size_t a_len, size_t lda, const T *b, size_t b_len, size_t ldb, T br, T bi,
+                   T *c, size_t c_len, size_t ldc) {
+  Check chk = cgemm_validate<T>(ta, tb, m, n, k, a_len, lda, b_len, ldb, c_len, ldc);
+  if (!chk.ok()) return fail(chk, nullptr);
+  if (m == 0 || n == 0) return RBLAS_OK;
+  const bool at = is_transposed(ta), bt = is_transposed(tb);
+  const size_t ea = cext(at ? k : m, at ? m : k, lda), eb = cext(bt ? n : k, bt ? k : n, ldb);
+  const size_t ec = cext(m, n, ldc);
+  CtxLease lease;
+  if (!lease.get()) return hip_fail(lease.status(), "stream create");
+  Ctx &ctx = *lease.get();
+  const size_t cs = 2 * sizeof(T);
+  RBLAS_HIP_TRY(ctx.ensure(0, ea * cs), "device buffer A");
+  RBLAS_HIP_TRY(ctx.ensure(1, eb * cs), "device buffer B");
+  RBLAS_HIP_TRY(ctx.ensure(2, ec * cs), "device buffer C");
+  RBLAS_HIP_TRY(ctx.ensure(3, cgemm_workspace_elems(m, n, k) * sizeof(T)), "device workspace");
+  hipStream_t st = ctx.stream;
+  if (ea) RBLAS_HIP_TRY(hipMemcpyAsync(ctx.buf[0], a, ea * cs, hipMemcpyHostToDevice, st), "copy A");
+  if (eb) RBLAS_HIP_TRY(hipMemcpyAsync(ctx.buf[1], b, eb * cs, hipMemcpyHostToDevice, st), "copy B");
+  RBLAS_HIP_TRY(hipMemcpyAsync(ctx.buf[2], c, ec * cs, hipMemcpyHostToDevice, st), "copy C");
+  CgemmProblem<T> p{ta, tb, (int64_t)m, (int64_t)n, (int64_t)k, ar, ai, br, bi,
+                    (const T *)ctx.buf[0], (int64_t)lda, (const T *)ctx.buf[1], (int64_t)ldb,
+                    (T *)ctx.buf[2], (int64_t)ldc};
+  RBLAS_HIP_TRY(launch_cgemm<T>(p, (T *)ctx.buf[3], st), "complex gemm launch");
+  RBLAS_HIP_TRY(hipMemcpyAsync(c, ctx.buf[2], ec * cs, hipMemcpyDeviceToHost, st), "copy C back");
+  RBLAS_HIP_TRY(hipStreamSynchronize(st), "complex gemm");
+  return RBLAS_OK;
+}
+
+template <typename T>
+int32_t cgemm_dev(int32_t ta, int32_t tb, size_t m, size_t n, size_t k, T ar, T ai, const T *a,
+                  size_t a_len, size_t lda, const T *b, size_t b_len, size_t ldb, T br, T bi, T *c,
+                  size_t c_len, size_t ldc, void *stream) {
+  Check chk = cgemm_validate<T>(ta, tb, m, n, k, a_len, lda, b_len, ldb, c_len, ldc);
+  if (!chk.ok()) return fail(chk, nullptr);
+  if (m == 0 || n == 0) return RBLAS_OK;
+  hipStream_t st = (hipStream_t)stream;
+  void *ws = nullptr;
+  RBLAS_HIP_TRY(pool_alloc(&ws, cgemm_workspace_elems(m, n, k) * sizeof(T), st),
+                "complex gemm workspace");
+  CgemmProblem<T> p{ta, tb, (int64_t)m, (int64_t)n, (int64_t)k, ar, ai, br, bi, a, (int64_t)lda,
+                    b, (int64_t)ldb, c, (int64_t)ldc};
+  hipError_t e = launch_cgemm<T>(p, (T *)ws, st);
+  hipFreeAsync(ws, st);
+  RBLAS_HIP_TRY(e, "complex gemm launch");
+  return RBLAS_OK;
+}
+
+}  // namespace
+}  // namespace rblas
+
+using namespace rblas;
+
+extern "C" {
+
+int32_t rblas_cgemm(int32_t transa, int32_t transb, size_t m, size_t n, size_t k,
+                    rblas_complex32_t alpha, const rblas_complex32_t *a, size_t a_len, size_t lda,
+                    const rblas_complex32_t *b, size_t b_len, size_t ldb, rblas_complex32_t beta,
+                    rblas_complex32_t *c, size_t c_len, size_t ldc) {
+  return cgemm_host<float>(transa, transb, m, n, k, alpha.re, alpha.im, (const float *)a, a_len,
+                           lda, (const float *)b, b_len, ldb, beta.re, beta.im, (float *)c, c_len,
+                           ldc);
+}
+
+int32_t rblas_zgemm(int32_t transa, int32_t transb, size_t m, size_t n, size_t k,
+                    rblas_complex64_t alpha, const rblas_complex64_t *a, size_t a_len, size_t lda,
+                    const rblas_complex64_t *b, size_t b_len, size_t ldb, rblas_complex64_t beta,
+                    rblas_complex64_t *c, size_t c_len, size_t ldc) {
+  return cgemm_host<double>(transa, transb, m, n, k, alpha.re, alpha.im, (const double *)a, a_len,
+                            lda, (const double *)b, b_len, ldb, beta.re, beta.im, (double *)c,
+                            c_len, ldc);
+}
+
+int32_t rblas_cgemm_dev(int32_t transa, int32_t transb, size_t m, size_t n, size_t k,
+                        rblas_complex32_t alpha, const rblas_complex32_t *a, size_t a_len,
+                        size_t lda, const rblas_complex32_t *b, size_t b_len, size_t ldb,
+                        rblas_complex32_t beta, rblas_complex32_t *c, size_t c_len, size_t ldc,
+                        void *stream) {
+  return cgemm_dev<float>(transa, transb, m, n, k, alpha.re, alpha.im, (const float *)a, a_len,
+                          lda, (const float *)b, b_len, ldb, beta.re, beta.im, (float *)c, c_len,
+                          ldc, stream);
+}
+
+int32_t rblas_zgemm_dev(int32_t transa, int32_t transb, size_t m, size_t n, size_t k,
+                        rblas_complex64_t alpha, const rblas_complex64_t *a, size_t a_len,
+                        size_t lda, const rblas_complex64_t *b, size_t b_len, size_t ldb,
+                        rblas_complex64_t beta, rblas_complex64_t *c, size_t c_len, size_t ldc,
+                        void *stream) {
+  return cgemm_dev<double>(transa, transb, m, n, k, alpha.re, alpha.im, (const double *)a, a_len,
+                           lda, (const double *)b, b_len, ldb, beta.re, beta.im, (double *)c,
+                           c_len, ldc, stream);
+}
+
+}  // extern "C"

@@ -248,6 +248,24 @@ __device__ __forceinline__ void tile_coords(int tiles_m, int tiles_n, int &tm, i
   tn = in_group / gm;
 }
 
+// Triangle tiles for syrk: workgroup t takes the t-th tile of the tile-level
+// triangle in row order (row r holds r+1 tiles): lower -> (r, c <= r),
+// upper -> (c <= r, r).
+__device__ __forceinline__ void tri_coords(int tri, int &tm, int &tn) {
+  const int t = blockIdx.x;
+  int r = (int)((__builtin_sqrtf(8.f * (float)t + 1.f) - 1.f) * 0.5f);
+  while ((r + 1) * (r + 2) / 2 <= t) ++r;
+  while (r * (r + 1) / 2 > t) --r;
+  const int c = t - r * (r + 1) / 2;
+  if (tri == 2) {
+    tm = r;
+    tn = c;
+  } else {
+    tm = c;
+    tn = r;
+  }
+}
+
 // Workgroup barrier for the LDS pipeline: wait until at most N of this
 // wave's vector-memory instructions (the DMA loads of later tiles) are
 // outstanding and all its LDS stores are done, then s_barrier. Unlike
@@ -377,7 +395,10 @@ __global__ __launch_bounds__(kThreads) void gemm_tile_kernel(GemmProblem<T> p) {
   const int tiles_m = (M + kTile - 1) / kTile;
   const int tiles_n = (N + kTile - 1) / kTile;
   int tm, tn;
-  tile_coords(tiles_m, tiles_n, tm, tn);
+  if (p.tri == 0)
+    tile_coords(tiles_m, tiles_n, tm, tn);
+  else
+    tri_coords(p.tri, tm, tn);
   const int m0 = tm * kTile, n0 = tn * kTile;
 
   tg.tid = threadIdx.x;
@@ -428,16 +449,30 @@ __global__ __launch_bounds__(kThreads) void gemm_tile_kernel(GemmProblem<T> p) {
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int gj = n0 + tg.wn + 16 * j + Mfma<T>::row(tg.lane, r);
-        cv[r] = (read_c && gj < N) ? p.c[gi + (int64_t)gj * p.ldc] : T(0);
+        const bool in = gj < N && !((p.tri == 1 && gi > gj) || (p.tri == 2 && gi < gj));
+        cv[r] = (read_c && in) ? p.c[gi + (int64_t)gj * p.ldc] : T(0);
       }
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int gj = n0 + tg.wn + 16 * j + Mfma<T>::row(tg.lane, r);
         if (gj >= N) continue;
+        if ((p.tri == 1 && gi > gj) || (p.tri == 2 && gi < gj)) continue;
         const T v = tg.acc[i][j][r];
         p.c[gi + (int64_t)gj * p.ldc] = read_c ? __builtin_fma(alpha, v, beta * cv[r]) : alpha * v;
       }
     }
+  }
+}
+
+template <typename T>
+__global__ void scale_tri_kernel(int64_t n, int tri, T beta, T *c, int64_t ldc) {
+  const int64_t total = n * n;
+  for (int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; idx < total;
+       idx += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t j = idx / n, i = idx - j * n;
+    if ((tri == 1 && i > j) || (tri == 2 && i < j)) continue;
+    T *cp = c + i + j * ldc;
+    *cp = beta == T(0) ? T(0) : beta * *cp;
   }
 }
 
@@ -465,7 +500,12 @@ bool gemm_fits(const GemmProblem<T> &p) {
 template <typename T>
 hipError_t launch_gemm(const GemmProblem<T> &p, hipStream_t stream) {
   if (!gemm_fits(p)) return hipErrorInvalidValue;
-  const int64_t tiles = ((p.m + kTile - 1) / kTile) * ((p.n + kTile - 1) / kTile);
+  int64_t tiles = ((p.m + kTile - 1) / kTile) * ((p.n + kTile - 1) / kTile);
+  if (p.tri != 0) {
+    if (p.m != p.n) return hipErrorInvalidValue;
+    const int64_t t = (p.m + kTile - 1) / kTile;
+    tiles = t * (t + 1) / 2;
+  }
   dim3 grid((unsigned)tiles), block(kThreads);
   if (!p.ta && !p.tb)
     hipLaunchKernelGGL((gemm_tile_kernel<T, false, false>), grid, block, 0, stream, p);
@@ -489,6 +529,18 @@ hipError_t launch_scale(int64_t m, int64_t n, T beta, T *c, int64_t ldc, hipStre
   return hipGetLastError();
 }
 
+template <typename T>
+hipError_t launch_scale_tri(int64_t n, int tri, T beta, T *c, int64_t ldc, hipStream_t stream) {
+  int64_t blocks = (n * n + 255) / 256;
+  if (blocks > 65536) blocks = 65536;
+  if (blocks < 1) blocks = 1;
+  hipLaunchKernelGGL((scale_tri_kernel<T>), dim3((unsigned)blocks), dim3(256), 0, stream, n, tri,
+                     beta, c, ldc);
+  return hipGetLastError();
+}
+
+template hipError_t launch_scale_tri<float>(int64_t, int, float, float *, int64_t, hipStream_t);
+template hipError_t launch_scale_tri<double>(int64_t, int, double, double *, int64_t, hipStream_t);
 template hipError_t launch_gemm<float>(const GemmProblem<float> &, hipStream_t);
 template hipError_t launch_gemm<double>(const GemmProblem<double> &, hipStream_t);
 template hipError_t launch_scale<float>(int64_t, int64_t, float, float *, int64_t, hipStream_t);

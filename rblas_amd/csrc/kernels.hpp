@@ -22,6 +22,9 @@ struct GemmProblem {
   int64_t ldb;
   T *c;
   int64_t ldc;
+  // 0: all of C. 1 / 2: only the upper (i <= j) / lower (i >= j) triangle of
+  // a square C is computed and written (syrk); only tiles touching it run.
+  int tri = 0;
 };
 
 // C := alpha * op(A) * op(B) + beta * C on MFMA tiles. Requires m, n > 0,
@@ -35,6 +38,9 @@ hipError_t launch_gemm(const GemmProblem<T> &p, hipStream_t stream);
 template <typename T>
 hipError_t launch_scale(int64_t m, int64_t n, T beta, T *c, int64_t ldc,
                         hipStream_t stream);
+// The same on one triangle of an n x n C (tri 1 = upper, 2 = lower).
+template <typename T>
+hipError_t launch_scale_tri(int64_t n, int tri, T beta, T *c, int64_t ldc, hipStream_t stream);
 
 // SGEMM through bf16 splits (planes = 2: BF16x3, planes = 3: BF16x6). Same
 // preconditions as launch_gemm. `workspace` holds the split planes and must
@@ -43,6 +49,43 @@ hipError_t launch_scale(int64_t m, int64_t n, T beta, T *c, int64_t ldc,
 size_t sgemm_split_workspace(int64_t m, int64_t n, int64_t k, int planes);
 hipError_t launch_sgemm_split(const GemmProblem<float> &p, int planes, void *workspace,
                               hipStream_t stream);
+
+// Complex gemm by 4M (interleaved {re, im} storage, column-major). op codes:
+// 0 none, 1 transpose, 2 conjugate transpose. Requires m, n > 0. `ws` holds
+// cgemm_workspace_elems(m, n, k) elements of T.
+template <typename T>
+struct CgemmProblem {
+  int ta, tb;
+  int64_t m, n, k;
+  T alpha_re, alpha_im, beta_re, beta_im;
+  const T *a;
+  int64_t lda;
+  const T *b;
+  int64_t ldb;
+  T *c;
+  int64_t ldc;
+};
+size_t cgemm_workspace_elems(int64_t m, int64_t n, int64_t k);
+template <typename T>
+hipError_t launch_cgemm(const CgemmProblem<T> &p, T *ws, hipStream_t stream);
+
+// Triangular solve in place on B (m x n, column-major). `upper` is the
+// effective triangle of op(A) (uplo flipped by trans). Requires the whole
+// op(A) diagonal to be readable and m, n > 0 handled by the caller's checks.
+// `ws` holds trsm_workspace_bytes() (the f64 inverses of the diagonal blocks).
+template <typename T>
+struct TrsmProblem {
+  bool left, upper, trans, unit;
+  int64_t m, n;
+  T alpha;
+  const T *a;
+  int64_t lda;
+  T *b;
+  int64_t ldb;
+};
+size_t trsm_workspace_bytes(int64_t m, int64_t n, bool left);
+template <typename T>
+hipError_t launch_trsm(const TrsmProblem<T> &p, void *ws, hipStream_t stream);
 
 // Level-1 (HBM-bound). `scratch` holds per-block partials, at least
 // level1_scratch_bytes(); the result is written to device memory `out`.

@@ -1,0 +1,254 @@
+// Blocked triangular solve on gfx950 (level3::trsm, level3.rs:390-520):
+// op(A) X = alpha B (Left) or X op(A) = alpha B (Right), in place in B.
+//
+// rblas substitutes element by element. Here:
+//   1. B := alpha B when alpha != 1 (level3.rs:403-409; every entry, so a
+//      zero alpha keeps NaN).
+//   2. trsm_diag_inv inverts every 64 x 64 diagonal block of op(A) at once,
+//      in f64 for f32 and f64 operands alike (Unit: the stored diagonal is
+//      not read).
+//   3. The diagonal range is solved by recursive halving: solve the half
+//      that comes first in substitution order, subtract its contribution from
+//      the other half with one MFMA gemm (k = the first half's length, so the
+//      bulk of the flops runs at large k), solve the other half. A 64-row
+//      leaf is solved by trsm_apply_* in place: X_b = inv_b * B_b (Left) or
+//      B_b * inv_b (Right) in f64, summing only over the triangle, so a
+//      non-finite B entry reaches exactly the entries that depend on it in
+//      rblas's substitution (no 0 * NaN from structural zeros).
+#include "kernels.hpp"
+
+namespace rblas {
+namespace {
+
+constexpr int kNB = 64;
+
+// op(A)(r, c) of the stored column-major A.
+template <typename T>
+__device__ __forceinline__ T op_a(const T *a, int64_t lda, bool trans, int64_t r, int64_t c) {
+  return trans ? a[c + r * lda] : a[r + c * lda];
+}
+
+// Inverse of diagonal block b of op(A) (rows/cols b*64 .. min(q, b*64+64)),
+// written column-major 64 x 64 (ld 64, f64) at inv + b*64*64; entries outside
+// the effective triangle and outside the block are zero. Thread j builds
+// column j by substitution against the unit vector e_j.
+template <typename T>
+__global__ __launch_bounds__(kNB) void trsm_diag_inv(const T *a, int64_t lda, bool trans,
+                                                     bool upper, bool unit, int64_t q,
+                                                     double *inv) {
+  __shared__ double L[kNB][kNB + 1];
+  __shared__ double X[kNB][kNB + 1];
+  const int64_t r0 = (int64_t)blockIdx.x * kNB;
+  const int nb = (int)((q - r0) < kNB ? (q - r0) : kNB);
+  const int t = threadIdx.x;
+  for (int idx = t; idx < kNB * kNB; idx += kNB) {
+    const int r = idx % kNB, c = idx / kNB;
+    double v = 0.0;
+    const bool in_tri = upper ? r <= c : r >= c;
+    if (r < nb && c < nb && in_tri)
+      v = (unit && r == c) ? 1.0 : (double)op_a(a, lda, trans, r0 + r, r0 + c);
+    L[r][c] = v;
+    X[r][c] = 0.0;
+  }
+  __syncthreads();
+  const int j = t;
+  if (j < nb) {
+    X[j][j] = 1.0 / L[j][j];
+    if (!upper) {
+      for (int i = j + 1; i < nb; ++i) {
+        double s = 0.0;
+        for (int kk = j; kk < i; ++kk) s = __builtin_fma(L[i][kk], X[kk][j], s);
+        X[i][j] = -s / L[i][i];
+      }
+    } else {
+      for (int i = j - 1; i >= 0; --i) {
+        double s = 0.0;
+        for (int kk = i + 1; kk <= j; ++kk) s = __builtin_fma(L[i][kk], X[kk][j], s);
+        X[i][j] = -s / L[i][i];
+      }
+    }
+  }
+  __syncthreads();
+  double *out = inv + (int64_t)blockIdx.x * kNB * kNB;
+  for (int idx = t; idx < kNB * kNB; idx += kNB) out[idx] = X[idx % kNB][idx / kNB];
+}
+
+// Left leaf: B[r0 + i, col] := sum_{j in tri(i)} inv[i][j] * B[r0 + j, col]
+// for i, j < nb and the 64 columns of this workgroup. 256 threads: lane row
+// i = t % 64 (coalesced on column-major B), columns t / 64 + 4 r.
+template <typename T>
+__global__ __launch_bounds__(256) void trsm_apply_left(const double *inv, bool upper, int nb,
+                                                       int64_t n, T *b, int64_t ldb) {
+  __shared__ double invT[kNB][kNB];  // invT[j][i] = inv(i, j)
+  __shared__ double Bs[kNB][kNB];    // Bs[c][j] = B(r0 + j, col0 + c)
+  const int t = threadIdx.x, i = t % kNB, c0 = t / kNB;
+  const int64_t col0 = (int64_t)blockIdx.x * kNB;
+  const int ncol = (int)((n - col0) < kNB ? (n - col0) : kNB);
+  for (int idx = t; idx < kNB * kNB; idx += 256) invT[idx / kNB][idx % kNB] = inv[idx];
+  for (int r = 0; r < 16; ++r) {
+    const int c = c0 + 4 * r;
+    Bs[c][i] = (c < ncol && i < nb) ? (double)b[i + (col0 + c) * ldb] : 0.0;
+  }
+  __syncthreads();
+  double acc[16];
+#pragma unroll
+  for (int r = 0; r < 16; ++r) acc[r] = 0.0;
+  for (int j = 0; j < nb; ++j) {
+    const double w = invT[j][i];
+    const bool on = upper ? j >= i : j <= i;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      const double v = __builtin_fma(w, Bs[c0 + 4 * r][j], acc[r]);
+      acc[r] = on ? v : acc[r];
+    }
+  }
+  if (i < nb) {
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      const int c = c0 + 4 * r;
+      if (c < ncol) b[i + (col0 + c) * ldb] = (T)acc[r];
+    }
+  }
+}
+
+// Right leaf: B[row, r0 + j] := sum_{p in tri(j)} B[row, r0 + p] * inv[p][j]
+// for the 64 rows of this workgroup. Lane row i = t % 64, columns
+// j = t / 64 + 4 r (uniform across a wavefront, so inv reads broadcast).
+template <typename T>
+__global__ __launch_bounds__(256) void trsm_apply_right(const double *inv, bool upper, int nb,
+                                                        int64_t m, T *b, int64_t ldb) {
+  __shared__ double inv_s[kNB][kNB];  // inv_s[j][p] = inv(p, j)
+  __shared__ double Bs[kNB][kNB];     // Bs[p][i] = B(row0 + i, r0 + p)
+  const int t = threadIdx.x, i = t % kNB, c0 = t / kNB;
+  const int64_t row0 = (int64_t)blockIdx.x * kNB;
+  const bool row_ok = row0 + i < m;
+  for (int idx = t; idx < kNB * kNB; idx += 256) inv_s[idx / kNB][idx % kNB] = inv[idx];
+  for (int r = 0; r < 16; ++r) {
+    const int p = c0 + 4 * r;
+    Bs[p][i] = (row_ok && p < nb) ? (double)b[row0 + i + p * ldb] : 0.0;
+  }
+  __syncthreads();
+  double acc[16];
+#pragma unroll
+  for (int r = 0; r < 16; ++r) acc[r] = 0.0;
+  for (int p = 0; p < nb; ++p) {
+    const double v = Bs[p][i];
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      const int j = c0 + 4 * r;
+      const bool on = upper ? p <= j : p >= j;
+      if (on) acc[r] = __builtin_fma(v, inv_s[j][p], acc[r]);
+    }
+  }
+  if (row_ok) {
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      const int j = c0 + 4 * r;
+      if (j < nb) b[row0 + i + j * ldb] = (T)acc[r];
+    }
+  }
+}
+
+// B := alpha * B over m x n.
+template <typename T>
+__global__ void mul_kernel(int64_t m, int64_t n, T alpha, T *b, int64_t ldb) {
+  const int64_t total = m * n;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t j = e / m, i = e - j * m;
+    b[i + j * ldb] = alpha * b[i + j * ldb];
+  }
+}
+
+unsigned grid_for(int64_t total) {
+  int64_t b = (total + 255) / 256;
+  if (b > 16384) b = 16384;
+  if (b < 1) b = 1;
+  return (unsigned)b;
+}
+
+// Pointer to op(A)(r0, c0) as a gemm operand (transposed storage when trans).
+template <typename T>
+const T *sub(const T *a, int64_t lda, bool trans, int64_t r0, int64_t c0) {
+  return trans ? a + c0 + r0 * lda : a + r0 + c0 * lda;
+}
+
+template <typename T>
+struct Solver {
+  const TrsmProblem<T> &p;
+  const double *inv;
+  hipStream_t st;
+  bool forward;  // substitution runs from index 0 up
+
+  // Solve the diagonal range [lo, lo + len) (lo a multiple of 64), every
+  // contribution from outside it having been subtracted already.
+  hipError_t range(int64_t lo, int64_t len) {
+    if (len <= kNB) {
+      const double *ib = inv + (lo / kNB) * kNB * kNB;
+      if (p.left)
+        hipLaunchKernelGGL((trsm_apply_left<T>), dim3((unsigned)((p.n + kNB - 1) / kNB)),
+                           dim3(256), 0, st, ib, p.upper, (int)len, p.n, p.b + lo, p.ldb);
+      else
+        hipLaunchKernelGGL((trsm_apply_right<T>), dim3((unsigned)((p.m + kNB - 1) / kNB)),
+                           dim3(256), 0, st, ib, p.upper, (int)len, p.m, p.b + lo * p.ldb,
+                           p.ldb);
+      return hipGetLastError();
+    }
+    const int64_t h = ((len / 2 + kNB - 1) / kNB) * kNB;
+    int64_t f_lo = lo, f_len = h, s_lo = lo + h, s_len = len - h;
+    if (!forward) {
+      f_lo = lo + h;
+      f_len = len - h;
+      s_lo = lo;
+      s_len = h;
+    }
+    hipError_t e = range(f_lo, f_len);
+    if (e != hipSuccess) return e;
+    if (p.left) {
+      // B[second rows] -= op(A)[second, first] * X[first rows]
+      GemmProblem<T> u{p.trans, false, s_len, p.n, f_len, T(-1), T(1),
+                       sub(p.a, p.lda, p.trans, s_lo, f_lo), p.lda, p.b + f_lo, p.ldb,
+                       p.b + s_lo, p.ldb};
+      e = launch_gemm<T>(u, st);
+    } else {
+      // B[:, second] -= X[:, first] * op(A)[first, second]
+      GemmProblem<T> u{false, p.trans, p.m, s_len, f_len, T(-1), T(1), p.b + f_lo * p.ldb, p.ldb,
+                       sub(p.a, p.lda, p.trans, f_lo, s_lo), p.lda, p.b + s_lo * p.ldb, p.ldb};
+      e = launch_gemm<T>(u, st);
+    }
+    if (e != hipSuccess) return e;
+    return range(s_lo, s_len);
+  }
+};
+
+}  // namespace
+
+size_t trsm_workspace_bytes(int64_t m, int64_t n, bool left) {
+  const int64_t q = left ? m : n;
+  const int64_t nblk = (q + kNB - 1) / kNB;
+  return (size_t)(nblk * kNB * kNB) * sizeof(double);
+}
+
+template <typename T>
+hipError_t launch_trsm(const TrsmProblem<T> &p, void *ws, hipStream_t st) {
+  const int64_t m = p.m, n = p.n;
+  if (m == 0 || n == 0) return hipSuccess;
+  const int64_t q = p.left ? m : n;
+  const int64_t nblk = (q + kNB - 1) / kNB;
+  double *inv = (double *)ws;
+  if (p.alpha != T(1))
+    hipLaunchKernelGGL((mul_kernel<T>), dim3(grid_for(m * n)), dim3(256), 0, st, m, n, p.alpha,
+                       p.b, p.ldb);
+  hipLaunchKernelGGL((trsm_diag_inv<T>), dim3((unsigned)nblk), dim3(kNB), 0, st, p.a, p.lda,
+                     p.trans, p.upper, p.unit, q, inv);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  // Left+lower and Right+upper substitute from index 0 up, the others down.
+  Solver<T> s{p, inv, st, p.left ? !p.upper : p.upper};
+  return s.range(0, q);
+}
+
+template hipError_t launch_trsm<float>(const TrsmProblem<float> &, void *, hipStream_t);
+template hipError_t launch_trsm<double>(const TrsmProblem<double> &, void *, hipStream_t);
+
+}  // namespace rblas

@@ -120,3 +120,40 @@ def sgemm_ex_dev(layout, transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c
                                        C.c_float(beta), c.ptr, c.size, ldc, int(compute),
                                        stream.handle if stream is not None else None)
     raise_status(st)
+
+
+def syrk(uplo, trans, n, k, alpha, a, lda, beta, c, ldc) -> None:
+    """C_uplo := alpha*op(A)*op(A)^T + beta*C_uplo (level3.rs:236-294)."""
+    pre = _prefix(a, c)
+    st = getattr(_ffi.lib(), f"rblas_{pre}syrk")(
+        int(uplo), int(trans), n, k, _scalar(pre, alpha), a.ctypes.data, a.size, lda,
+        _scalar(pre, beta), c.ctypes.data, c.size, ldc)
+    raise_status(st)
+
+
+def syrk_dev(uplo, trans, n, k, alpha, a, lda, beta, c, ldc, stream=None) -> None:
+    """syrk on HBM-resident operands, enqueued on `stream`."""
+    pre = _dev_prefix(a, c)
+    st = getattr(_ffi.lib(), f"rblas_{pre}syrk_dev")(
+        int(uplo), int(trans), n, k, _scalar(pre, alpha), a.ptr, a.size, lda,
+        _scalar(pre, beta), c.ptr, c.size, ldc, stream.handle if stream is not None else None)
+    raise_status(st)
+
+
+def trsm(side, uplo, trans, diag, m, n, alpha, a, lda, b, ldb) -> None:
+    """Solve op(A) X = alpha B or X op(A) = alpha B in place in b
+    (level3.rs:390-520)."""
+    pre = _prefix(a, b)
+    st = getattr(_ffi.lib(), f"rblas_{pre}trsm")(
+        int(side), int(uplo), int(trans), int(diag), m, n, _scalar(pre, alpha), a.ctypes.data,
+        a.size, lda, b.ctypes.data, b.size, ldb)
+    raise_status(st)
+
+
+def trsm_dev(side, uplo, trans, diag, m, n, alpha, a, lda, b, ldb, stream=None) -> None:
+    """trsm on HBM-resident operands, enqueued on `stream`."""
+    pre = _dev_prefix(a, b)
+    st = getattr(_ffi.lib(), f"rblas_{pre}trsm_dev")(
+        int(side), int(uplo), int(trans), int(diag), m, n, _scalar(pre, alpha), a.ptr, a.size,
+        lda, b.ptr, b.size, ldb, stream.handle if stream is not None else None)
+    raise_status(st)
