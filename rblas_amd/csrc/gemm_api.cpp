@@ -75,27 +75,39 @@ inline int split_planes(int32_t compute) {
   return compute == RBLAS_COMPUTE_BF16X3 ? 2 : compute == RBLAS_COMPUTE_BF16X6 ? 3 : 0;
 }
 
+// The shape-only problem of a normalized call (pointers unset).
+template <typename T>
+GemmProblem<T> shape_of(const ColMajorCall<T> &cc) {
+  return GemmProblem<T>{cc.at, cc.bt, (int64_t)cc.m, (int64_t)cc.n, (int64_t)cc.k, cc.alpha,
+                        cc.beta, nullptr, (int64_t)cc.lda, nullptr, (int64_t)cc.ldb, nullptr,
+                        (int64_t)cc.ldc};
+}
+
+// Device workspace of a call: the bf16 planes of a split SGEMM, or the k-split
+// partials of the MFMA gemm's launch shape.
 template <typename T>
 size_t workspace_bytes(const ColMajorCall<T> &cc, int planes) {
-  if (planes == 0 || !runs_product(cc)) return 0;
-  return sgemm_split_workspace((int64_t)cc.m, (int64_t)cc.n, (int64_t)cc.k, planes);
+  if (!runs_product(cc)) return 0;
+  if (planes) return sgemm_split_workspace((int64_t)cc.m, (int64_t)cc.n, (int64_t)cc.k, planes);
+  const GemmProblem<T> p = shape_of(cc);
+  return gemm_workspace_bytes<T>(p, gemm_config<T>(p));
 }
 
 template <typename T>
 hipError_t launch_product(const GemmProblem<T> &p, int planes, void *ws, hipStream_t st) {
-  return launch_gemm<T>(p, st);
+  return launch_gemm_cfg<T>(p, gemm_config<T>(p), ws, st);
 }
 template <>
 hipError_t launch_product<float>(const GemmProblem<float> &p, int planes, void *ws,
                                  hipStream_t st) {
   if (planes) return launch_sgemm_split(p, planes, ws, st);
-  return launch_gemm<float>(p, st);
+  return launch_gemm_cfg<float>(p, gemm_config<float>(p), ws, st);
 }
 
 // Enqueue the device work of a normalized call whose pointers are in HBM.
 // Mirrors gemm_contig's control flow (kernel/gemm.rs:198-205): nothing when
-// m or n is 0; beta-only update when k is 0 or alpha is 0. `ws` is the split
-// workspace (planes > 0 only).
+// m or n is 0; beta-only update when k is 0 or alpha is 0. `ws` holds
+// workspace_bytes(cc, planes).
 template <typename T>
 hipError_t enqueue(const ColMajorCall<T> &cc, const T *a, size_t lda, const T *b, size_t ldb, T *c,
                    size_t ldc, int planes, void *ws, hipStream_t st) {
@@ -125,7 +137,7 @@ int32_t gemm_host(const ColMajorCall<T> &cc, int planes = 0) {
   RBLAS_HIP_TRY(ctx.ensure(0, sa.elems * sizeof(T)), "device buffer A");
   RBLAS_HIP_TRY(ctx.ensure(1, sb.elems * sizeof(T)), "device buffer B");
   RBLAS_HIP_TRY(ctx.ensure(2, sc.elems * sizeof(T)), "device buffer C");
-  RBLAS_HIP_TRY(ctx.ensure(3, workspace_bytes(cc, planes)), "device split workspace");
+  RBLAS_HIP_TRY(ctx.ensure(3, workspace_bytes(cc, planes)), "device workspace");
   T *da = (T *)ctx.buf[0], *db = (T *)ctx.buf[1], *dc = (T *)ctx.buf[2];
   hipStream_t st = ctx.stream;
   if (need_ab) {
@@ -184,7 +196,7 @@ int32_t gemm_dev_entry(int32_t layout, int32_t ta, int32_t tb, size_t m, size_t 
   hipStream_t st = (hipStream_t)stream;
   const size_t wsb = workspace_bytes(cc, planes);
   void *ws = nullptr;
-  if (wsb) RBLAS_HIP_TRY(pool_alloc(&ws, wsb, st), "split workspace");
+  if (wsb) RBLAS_HIP_TRY(pool_alloc(&ws, wsb, st), "gemm workspace");
   hipError_t e = enqueue(cc, cc.a, cc.lda, cc.b, cc.ldb, cc.c, cc.ldc, planes, ws, st);
   if (ws) hipFreeAsync(ws, st);
   RBLAS_HIP_TRY(e, "gemm launch");

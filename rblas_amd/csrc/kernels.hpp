@@ -27,11 +27,35 @@ struct GemmProblem {
   int tri = 0;
 };
 
+// Compute units of one MI355X (8 XCDs x 32).
+constexpr int kCUs = 256;
+
+// Launch shape of one MFMA gemm: C tile side (64 or 128) and the number of
+// k splits (1 = none). A pure function of the problem's shape, so a problem
+// gives the same bits whenever and with whatever it runs.
+struct GemmConfig {
+  int tile;
+  int splits;
+};
+
 // C := alpha * op(A) * op(B) + beta * C on MFMA tiles. Requires m, n > 0,
 // k > 0 and alpha != 0 (the host routes the other cases to gemm_scale).
 // beta == 0 never reads C (rblas stores exact zeros, kernel/gemm.rs:328-331).
+// launch_gemm never splits k and needs no workspace; launch_gemm_cfg runs
+// `cfg`, whose k splits need gemm_workspace_bytes(p, cfg) bytes at `ws`
+// (partials and per-tile counters; free again once the stream has run it).
 template <typename T>
 hipError_t launch_gemm(const GemmProblem<T> &p, hipStream_t stream);
+template <typename T>
+GemmConfig gemm_config(const GemmProblem<T> &p);
+template <typename T>
+size_t gemm_workspace_bytes(const GemmProblem<T> &p, GemmConfig cfg);
+template <typename T>
+hipError_t launch_gemm_cfg(const GemmProblem<T> &p, GemmConfig cfg, void *ws,
+                           hipStream_t stream);
+// Whether every in-tile operand offset fits the 31-bit buffer addressing.
+template <typename T>
+bool gemm_fits(const GemmProblem<T> &p);
 
 // C := beta * C over an m x n block (rblas apply_beta, kernel/gemm.rs:322-338):
 // beta == 0 stores exact zeros, beta == 1 is a no-op the host skips.

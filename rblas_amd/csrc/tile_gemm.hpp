@@ -1,0 +1,527 @@
+// MFMA tile machinery shared by the single-problem GEMM (gemm_mfma.hip) and
+// the grouped persistent GEMM (grouped.hip). Device code only; every
+// translation unit that includes it gets its own LDS stages.
+//
+// One workgroup (4 waves, 2 x 2) computes one BM x BN tile of
+// C := alpha * op(A) * op(B) + beta * C (column-major), BM, BN in {64, 128},
+// each wave a (BM/2) x (BN/2) block of 16 x 16 MFMA accumulators, over a
+// k range [kbeg, kend) (the whole k, or one split of it).
+//
+// LDS staging. One DMA instruction moves 64 lanes x 16 bytes = 64*V elements
+// (V = 16 / sizeof(T): 2 for f64, 4 for f32) into one contiguous 1 KiB LDS
+// block, so a TILE x BK operand tile (BK = 8V) is TILE/8 blocks, TILE/32 per
+// wave. Blocks are padded so that the four lane groups of a fragment read
+// fall on different banks:
+//   M/N-contiguous operand: block q holds tile k-rows q*R .. q*R+R-1
+//     (R = 1024 / (TILE * sizeof(T))) as [row][TILE], lane l carrying V
+//     consecutive m (or n) of one row; block stride 1024 + 128/V bytes.
+//   K-contiguous operand: block q holds the 8 m (or n) q*8 .. q*8+7, each as
+//     its BK k (128 bytes = eight 16-byte chunks of V k). Lanes 8r .. 8r+7
+//     fetch the eight chunks of one column, so each instruction reads whole
+//     cache lines; chunk c of column mn is stored in slot c ^ (mn % 8) of
+//     that column's 128 bytes, which keeps 16-lane fragment reads free of
+//     bank conflicts. Block stride 1040 bytes.
+// MFMA k-steps are assigned so that a lane's V consecutive k of the
+// K-contiguous layout feed V consecutive MFMAs: in k-step s = V*t + p the
+// lanes of group g = lane/16 use k = 4*V*t + V*g + p. Both operands follow
+// the same assignment, so every k is used exactly once, and the order in
+// which each C entry accumulates its k does not depend on BM or BN.
+//
+// A partial last k-tile is staged through registers with per-element masks
+// so that every k >= kend slot holds an exact zero in both operands. Rows or
+// columns past the m/n edge may hold any value: they only reach C entries
+// outside the matrix, which the epilogue does not store.
+//
+// The MFMA is fed op(B)^T as its first operand and op(A)^T as its second, so
+// the accumulator's lane index runs down a column of C: 16 consecutive lanes
+// hold 16 consecutive rows of one C column and the epilogue writes runs of
+// column-major C.
+#pragma once
+#include "kernels.hpp"
+
+namespace rblas {
+namespace {
+
+typedef __attribute__((address_space(3))) void *lds_void_ptr;
+
+template <typename T>
+struct Mfma;
+
+template <>
+struct Mfma<double> {
+  typedef double acc_t __attribute__((ext_vector_type(4)));
+  static __device__ __forceinline__ acc_t mma(double a, double b, acc_t c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  // Accumulator element r of lane l is row (l/16) + 4r of the 16 x 16 block
+  // (measured on gfx950: the f64 form interleaves rows).
+  static __device__ __forceinline__ int row(int lane, int r) { return (lane >> 4) + 4 * r; }
+};
+
+template <>
+struct Mfma<float> {
+  typedef float acc_t __attribute__((ext_vector_type(4)));
+  static __device__ __forceinline__ acc_t mma(float a, float b, acc_t c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+  // Accumulator element r of lane l is row 4(l/16) + r of the 16 x 16 block.
+  static __device__ __forceinline__ int row(int lane, int r) { return 4 * (lane >> 4) + r; }
+};
+
+constexpr int kThreads = 256;  // 4 waves of 64, 2 x 2 over the C tile
+constexpr int kMaxTile = 128;  // largest C tile side
+constexpr int kStages = 4;     // LDS pipeline depth
+constexpr int kOperandBytes = (kMaxTile / 8) * (1024 + 64);  // largest operand stage
+constexpr uint32_t kMasked = 0x80000000u;  // buffer offset past num_records: reads 0
+
+template <typename T>
+struct Geo {
+  static constexpr int V = 16 / (int)sizeof(T);   // elements per lane per DMA
+  static constexpr int BK = 8 * V;                 // k per tile: 16 (f64) or 32 (f32)
+  static constexpr int NT = BK / (4 * V);          // 2 groups of V k-steps per tile
+};
+
+// Byte stride between DMA blocks of one operand tile.
+template <typename T, bool MN_MAJOR>
+constexpr int block_bytes() {
+  return MN_MAJOR ? 1024 + 128 / Geo<T>::V : 1024 + 16;
+}
+
+// LDS: kStages stages of (A tile, B tile), one distinct object per stage so
+// the compiler can tell the stage being read from the stages being filled.
+__shared__ __attribute__((aligned(16))) char g_lds_a0[kOperandBytes];
+__shared__ __attribute__((aligned(16))) char g_lds_a1[kOperandBytes];
+__shared__ __attribute__((aligned(16))) char g_lds_a2[kOperandBytes];
+__shared__ __attribute__((aligned(16))) char g_lds_a3[kOperandBytes];
+__shared__ __attribute__((aligned(16))) char g_lds_b0[kOperandBytes];
+__shared__ __attribute__((aligned(16))) char g_lds_b1[kOperandBytes];
+__shared__ __attribute__((aligned(16))) char g_lds_b2[kOperandBytes];
+__shared__ __attribute__((aligned(16))) char g_lds_b3[kOperandBytes];
+
+template <int S>
+__device__ __forceinline__ char *lds_a() {
+  if constexpr (S == 0) return g_lds_a0;
+  else if constexpr (S == 1) return g_lds_a1;
+  else if constexpr (S == 2) return g_lds_a2;
+  else return g_lds_a3;
+}
+template <int S>
+__device__ __forceinline__ char *lds_b() {
+  if constexpr (S == 0) return g_lds_b0;
+  else if constexpr (S == 1) return g_lds_b1;
+  else if constexpr (S == 2) return g_lds_b2;
+  else return g_lds_b3;
+}
+
+// Raw buffer resource over [base, base + bytes), bytes clamped to 2^31 - 1 so
+// that kMasked is always out of range. The host guarantees every in-tile
+// offset is below that (gemm_fits).
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void *base, uint64_t bytes) {
+  const int n = bytes >= 0x7fffffffull ? 0x7fffffff : (int)bytes;
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), (short)0, n, 0x00020000);
+}
+
+template <typename T>
+__device__ __forceinline__ T buffer_load(__amdgpu_buffer_rsrc_t r, uint32_t off);
+template <>
+__device__ __forceinline__ double buffer_load<double>(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+  return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
+}
+template <>
+__device__ __forceinline__ float buffer_load<float>(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
+}
+
+// Byte offset of tile element (k, mn) inside one operand's LDS tile.
+template <typename T, bool MN_MAJOR, int TILE>
+__device__ __forceinline__ int lds_byte(int k, int mn) {
+  typedef Geo<T> G;
+  constexpr int bb = block_bytes<T, MN_MAJOR>();
+  constexpr int R = 1024 / (TILE * (int)sizeof(T));
+  if (MN_MAJOR) return (k / R) * bb + ((k % R) * TILE + mn) * (int)sizeof(T);
+  const int r = mn % 8;
+  return (mn / 8) * bb + r * 128 + (((k / G::V) ^ r) * 16) + (k % G::V) * (int)sizeof(T);
+}
+
+// One operand (op(A) as m x k or op(B)^T as n x k) of the tile loop.
+template <typename T, bool MN_MAJOR, int TILE>
+struct Operand {
+  typedef Geo<T> G;
+  static constexpr int NB = TILE / 32;  // DMA blocks per wave per k-tile
+  static constexpr int R = 1024 / (TILE * (int)sizeof(T));
+  const T *x;
+  const T *end;  // one past the last element the stored operand may touch
+  int64_t ld;
+  int mn0, lim;      // tile origin along m/n, valid extent along m/n
+  uint32_t off[NB];  // this lane's byte offset for each of its wave's DMA blocks
+  int block0;        // first DMA block owned by this wave
+
+  __device__ __forceinline__ void init(const T *x_, const T *end_, int64_t ld_, int mn0_,
+                                       int mn_total, int wave, int lane) {
+    x = x_;
+    end = end_;
+    ld = ld_;
+    mn0 = mn0_;
+    lim = mn_total - mn0_;
+    block0 = wave * NB;
+#pragma unroll
+    for (int j = 0; j < NB; ++j) {
+      const int q = block0 + j;
+      int k, mn;
+      if (MN_MAJOR) {
+        const int e = lane * G::V;
+        k = q * R + e / TILE;
+        mn = e % TILE;
+      } else {
+        const int r = lane / 8;            // column within the block
+        k = ((lane % 8) ^ r) * G::V;       // chunk stored in slot lane % 8
+        mn = q * 8 + r;
+      }
+      const int64_t el = MN_MAJOR ? mn + (int64_t)k * ld : k + (int64_t)mn * ld;
+      off[j] = (uint32_t)(el * (int64_t)sizeof(T));
+    }
+  }
+
+  __device__ __forceinline__ const T *origin(int k0) const {
+    return MN_MAJOR ? x + mn0 + (int64_t)k0 * ld : x + k0 + (int64_t)mn0 * ld;
+  }
+
+  // Full k-tile: NB DMA blocks per wave straight into LDS.
+  __device__ __forceinline__ void dma(char *lds, int k0) const {
+    const T *o = origin(k0);
+    const __amdgpu_buffer_rsrc_t rs =
+        make_rsrc(o, (uint64_t)((const char *)end - (const char *)o));
+#pragma unroll
+    for (int j = 0; j < NB; ++j)
+      __builtin_amdgcn_raw_ptr_buffer_load_lds(
+          rs, (lds_void_ptr)(lds + (block0 + j) * block_bytes<T, MN_MAJOR>()), 16, off[j], 0, 0,
+          0);
+  }
+
+  // Partial last k-tile: element-wise masked loads, written in the same
+  // layout; k >= kend and m/n past the edge store exact zeros.
+  __device__ __forceinline__ void masked(char *lds, int k0, int kend, int tid) const {
+    const T *o = origin(k0);
+    const __amdgpu_buffer_rsrc_t rs =
+        make_rsrc(o, (uint64_t)((const char *)end - (const char *)o));
+    const int krem = kend - k0;
+#pragma unroll
+    for (int s = 0; s < TILE * G::BK / kThreads; ++s) {
+      const int e = tid + kThreads * s;
+      int k, mn;
+      if (MN_MAJOR) {
+        mn = e % TILE;
+        k = e / TILE;
+      } else {
+        k = e % G::BK;
+        mn = e / G::BK;
+      }
+      const int64_t el = MN_MAJOR ? mn + (int64_t)k * ld : k + (int64_t)mn * ld;
+      const bool ok = k < krem && mn < lim;
+      *(T *)(lds + lds_byte<T, MN_MAJOR, TILE>(k, mn)) =
+          buffer_load<T>(rs, ok ? (uint32_t)(el * (int64_t)sizeof(T)) : kMasked);
+    }
+  }
+};
+
+// Map tile index t of a tiles_m x tiles_n grid to (tile_m, tile_n).
+// Workgroups are dispatched round-robin over the 8 XCDs, so consecutive ids
+// land on different L2s; ids are first regrouped so that each XCD owns a
+// contiguous run of tiles, then that run is walked in column groups of 8
+// tile rows so the workgroups resident on one XCD share A and B panels in its
+// L2.
+__device__ __forceinline__ void tile_coords(int t, int tiles_m, int tiles_n, int &tm, int &tn) {
+  constexpr int kXcds = 8;
+  constexpr int kGroupM = 8;
+  const int total = tiles_m * tiles_n;
+  if (total % kXcds == 0) t = (t % kXcds) * (total / kXcds) + t / kXcds;
+  const int per_group = kGroupM * tiles_n;
+  const int group = t / per_group;
+  const int first_m = group * kGroupM;
+  const int gm = min(tiles_m - first_m, kGroupM);
+  const int in_group = t % per_group;
+  tm = first_m + in_group % gm;
+  tn = in_group / gm;
+}
+
+// Triangle tiles for syrk: tile t of the tile-level triangle in row order
+// (row r holds r+1 tiles): lower -> (r, c <= r), upper -> (c <= r, r).
+__device__ __forceinline__ void tri_coords(int t, int tri, int &tm, int &tn) {
+  int r = (int)((__builtin_sqrtf(8.f * (float)t + 1.f) - 1.f) * 0.5f);
+  while ((r + 1) * (r + 2) / 2 <= t) ++r;
+  while (r * (r + 1) / 2 > t) --r;
+  const int c = t - r * (r + 1) / 2;
+  if (tri == 2) {
+    tm = r;
+    tn = c;
+  } else {
+    tm = c;
+    tn = r;
+  }
+}
+
+// Workgroup barrier for the LDS pipeline: wait until at most N of this
+// wave's vector-memory instructions (the DMA loads of later tiles) are
+// outstanding and all its LDS stores are done, then s_barrier. Unlike
+// __syncthreads(), whose release fence drains every outstanding load
+// (vmcnt(0)), this keeps the later stages in flight. The empty asm statements
+// stop the compiler from moving memory accesses across the barrier.
+template <int N>
+__device__ __forceinline__ void stage_barrier() {
+  static_assert(N >= 0 && N < 64, "vmcnt range");
+  asm volatile("" ::: "memory");
+  __builtin_amdgcn_s_waitcnt((N & 0xf) | ((N >> 4) << 14) | 0x0070);  // vmcnt(N) lgkmcnt(0)
+  __builtin_amdgcn_s_barrier();
+  asm volatile("" ::: "memory");
+}
+
+// One past the last element a stored rows x cols column-major operand with
+// leading dimension ld can touch.
+template <typename T>
+__device__ __forceinline__ const T *extent_end(const T *x, int64_t rows, int64_t cols,
+                                               int64_t ld) {
+  return x + (cols - 1) * ld + rows;
+}
+
+template <typename T, bool TA, bool TB, int BM, int BN>
+struct TileGemm {
+  typedef Geo<T> G;
+  typedef typename Mfma<T>::acc_t acc_t;
+  static constexpr bool A_MN = !TA;   // A NoTrans is M-contiguous
+  static constexpr bool B_MN = TB;    // B Trans is N-contiguous
+  static constexpr int FM = BM / 32;  // 16 x 16 blocks per wave along m
+  static constexpr int FN = BN / 32;  // ... along n
+  static constexpr int kPerTile = BM / 32 + BN / 32;  // DMA instructions per wave per k-tile
+
+  Operand<T, A_MN, BM> a;
+  Operand<T, B_MN, BN> b;
+  acc_t acc[FM][FN];
+  int kbeg, kend, ktiles, full_tiles, tid, lane, wm, wn;
+
+  // Set up the tile at (m0, n0) over k range [kb, ke) of problem p.
+  __device__ __forceinline__ void init(const GemmProblem<T> &p, int m0, int n0, int kb, int ke) {
+    tid = threadIdx.x;
+    lane = tid & 63;
+    const int wave = tid >> 6;
+    wm = (wave & 1) * (BM / 2);
+    wn = (wave >> 1) * (BN / 2);
+    kbeg = kb;
+    kend = ke;
+    ktiles = (ke - kb + G::BK - 1) / G::BK;
+    full_tiles = (ke - kb) / G::BK;
+    // A NoTrans is stored m x k; A Trans is stored k x m.
+    // B NoTrans is stored k x n; B Trans is stored n x k.
+    a.init(p.a, TA ? extent_end(p.a, p.k, p.m, p.lda) : extent_end(p.a, p.m, p.k, p.lda), p.lda,
+           m0, (int)p.m, wave, lane);
+    b.init(p.b, TB ? extent_end(p.b, p.n, p.k, p.ldb) : extent_end(p.b, p.k, p.n, p.ldb), p.ldb,
+           n0, (int)p.n, wave, lane);
+#pragma unroll
+    for (int i = 0; i < FM; ++i)
+#pragma unroll
+      for (int j = 0; j < FN; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
+  }
+
+  // Stage k-tile kt into the LDS stage (la, lb): DMA when full, masked loads
+  // otherwise.
+  __device__ __forceinline__ void issue(int kt, char *la, char *lb) {
+    if (kt >= ktiles) return;
+    const int k0 = kbeg + kt * G::BK;
+    if (kt < full_tiles) {
+      a.dma(la, k0);
+      b.dma(lb, k0);
+    } else {
+      a.masked(la, k0, kend, tid);
+      b.masked(lb, k0, kend, tid);
+    }
+  }
+
+  // Barrier after which k-tile kt is in LDS for every wave. The tiles issued
+  // after kt are kt+1 .. kt+ahead, kPerTile DMA instructions per wave each;
+  // vector memory instructions complete in order, so vmcnt(kPerTile * ahead)
+  // means kt's have landed. (If one of the later tiles was the masked tail,
+  // its register loads were consumed by its own LDS stores, which already
+  // waited for everything issued before them.)
+  __device__ __forceinline__ void sync_tile(int kt) {
+    const int ahead = min(kStages - 2, ktiles - 1 - kt);
+    if (ahead >= 2)
+      stage_barrier<2 * kPerTile>();
+    else if (ahead == 1)
+      stage_barrier<kPerTile>();
+    else
+      stage_barrier<0>();
+  }
+
+  template <int S>
+  __device__ __forceinline__ void compute() {
+    const char *As = lds_a<S>();
+    const char *Bs = lds_b<S>();
+    const int g = lane >> 4, c = lane & 15;
+    typedef T vec_t __attribute__((ext_vector_type(G::V)));
+#pragma unroll
+    for (int t = 0; t < G::NT; ++t) {
+      T fa[G::V][FM], fb[G::V][FN];
+      const int kb = 4 * G::V * t + G::V * g;  // this lane group's first k
+#pragma unroll
+      for (int i = 0; i < FM; ++i) {
+        const int mn = wm + 16 * i + c;
+        if constexpr (A_MN) {
+#pragma unroll
+          for (int p = 0; p < G::V; ++p)
+            fa[p][i] = *(const T *)(As + lds_byte<T, true, BM>(kb + p, mn));
+        } else {
+          const vec_t v = *(const vec_t *)(As + lds_byte<T, false, BM>(kb, mn));
+#pragma unroll
+          for (int p = 0; p < G::V; ++p) fa[p][i] = v[p];
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < FN; ++j) {
+        const int mn = wn + 16 * j + c;
+        if constexpr (B_MN) {
+#pragma unroll
+          for (int p = 0; p < G::V; ++p)
+            fb[p][j] = *(const T *)(Bs + lds_byte<T, true, BN>(kb + p, mn));
+        } else {
+          const vec_t v = *(const vec_t *)(Bs + lds_byte<T, false, BN>(kb, mn));
+#pragma unroll
+          for (int p = 0; p < G::V; ++p) fb[p][j] = v[p];
+        }
+      }
+#pragma unroll
+      for (int p = 0; p < G::V; ++p)
+#pragma unroll
+        for (int i = 0; i < FM; ++i)
+#pragma unroll
+          for (int j = 0; j < FN; ++j)
+            acc[i][j] = Mfma<T>::mma(fb[p][j], fa[p][i], acc[i][j]);
+    }
+  }
+
+  // Pipeline step for k-tile kt, which lives in stage S.
+  template <int S>
+  __device__ __forceinline__ void step(int kt) {
+    if (kt >= ktiles) return;
+    sync_tile(kt);  // tile kt visible to all waves; the stage of kt-1 is free
+    constexpr int N = (S + kStages - 1) % kStages;
+    issue(kt + kStages - 1, lds_a<N>(), lds_b<N>());
+    compute<S>();
+  }
+
+  // The whole k loop: prologue of 3 stages, then unrolled by the 4 stages.
+  __device__ __forceinline__ void mainloop() {
+    issue(0, lds_a<0>(), lds_b<0>());
+    issue(1, lds_a<1>(), lds_b<1>());
+    issue(2, lds_a<2>(), lds_b<2>());
+    for (int kt = 0; kt < ktiles; kt += kStages) {
+      step<0>(kt);
+      step<1>(kt + 1);
+      step<2>(kt + 2);
+      step<3>(kt + 3);
+    }
+  }
+
+  // Row of C (within the tile) held by this lane in block row i; column of
+  // element r of block column j.
+  __device__ __forceinline__ int row_of(int i) const { return wm + 16 * i + (lane & 15); }
+  __device__ __forceinline__ int col_of(int j, int r) const {
+    return wn + 16 * j + Mfma<T>::row(lane, r);
+  }
+
+  // C := alpha * acc + beta * C on the tile at (m0, n0); beta == 0 never
+  // reads C; tri 1 / 2 keeps only the upper / lower triangle. Each block's C
+  // values are loaded together before any of them is stored.
+  __device__ __forceinline__ void epilogue(const GemmProblem<T> &p, int m0, int n0) const {
+    const int M = (int)p.m, N = (int)p.n;
+    const T alpha = p.alpha, beta = p.beta;
+    const bool read_c = beta != T(0);
+#pragma unroll
+    for (int i = 0; i < FM; ++i) {
+      const int gi = m0 + row_of(i);
+      if (gi >= M) continue;
+#pragma unroll
+      for (int j = 0; j < FN; ++j) {
+        T cv[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int gj = n0 + col_of(j, r);
+          const bool in = gj < N && !((p.tri == 1 && gi > gj) || (p.tri == 2 && gi < gj));
+          cv[r] = (read_c && in) ? p.c[gi + (int64_t)gj * p.ldc] : T(0);
+        }
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int gj = n0 + col_of(j, r);
+          if (gj >= N) continue;
+          if ((p.tri == 1 && gi > gj) || (p.tri == 2 && gi < gj)) continue;
+          const T v = acc[i][j][r];
+          p.c[gi + (int64_t)gj * p.ldc] =
+              read_c ? __builtin_fma(alpha, v, beta * cv[r]) : alpha * v;
+        }
+      }
+    }
+  }
+
+  static constexpr int kAccPerThread = FM * FN * 4;
+
+  // Split-k: store this split's raw accumulators in slot `slot` of `ws`
+  // (kAccPerThread * kThreads values per slot, in register order, so the
+  // same thread of any workgroup running this tile reads them back).
+  __device__ __forceinline__ void store_partial(T *ws, int64_t slot) const {
+    T *base = ws + slot * (int64_t)(kAccPerThread * kThreads) + tid;
+#pragma unroll
+    for (int i = 0; i < FM; ++i)
+#pragma unroll
+      for (int j = 0; j < FN; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) base[((i * FN + j) * 4 + r) * kThreads] = acc[i][j][r];
+  }
+
+  // acc := sum over s = 0 .. splits-1 of slot (slot0 + s), added in that
+  // order (split `mine` uses its own registers, which hold the same values it
+  // stored): a fixed order, so the result does not depend on which
+  // workgroup finishes last.
+  __device__ __forceinline__ void reduce_partials(const T *ws, int64_t slot0, int splits,
+                                                  int mine) {
+    const T *base = ws + tid;
+    const int64_t stride = (int64_t)(kAccPerThread * kThreads);
+#pragma unroll
+    for (int i = 0; i < FM; ++i)
+#pragma unroll
+      for (int j = 0; j < FN; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int e = ((i * FN + j) * 4 + r) * kThreads;
+          const T own = acc[i][j][r];
+          T sum = mine == 0 ? own : base[slot0 * stride + e];
+          for (int s = 1; s < splits; ++s)
+            sum += (s == mine) ? own : base[(slot0 + s) * stride + e];
+          acc[i][j][r] = sum;
+        }
+  }
+};
+
+// Split-k hand-off for one tile: every split stores its partial, publishes
+// it (device-scope release) and counts itself in `counter`; the split that
+// counts last (acquire) reduces all partials in split order and applies the
+// epilogue, then resets the counter to 0 for the next use. Returns true in
+// the workgroup that must run the epilogue.
+template <typename TG, typename T>
+__device__ __forceinline__ bool split_handoff(TG &tg, T *ws, int64_t slot0, int splits, int mine,
+                                              int *counter) {
+  __shared__ int s_last;
+  tg.store_partial(ws, slot0 + mine);
+  __threadfence();
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const int prev = atomicAdd(counter, 1);
+    s_last = prev == splits - 1;
+  }
+  __syncthreads();
+  if (!s_last) return false;
+  __threadfence();
+  tg.reduce_partials(ws, slot0, splits, mine);
+  if (threadIdx.x == 0) atomicExch(counter, 0);
+  return true;
+}
+
+}  // namespace
+}  // namespace rblas

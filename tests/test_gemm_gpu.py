@@ -209,3 +209,28 @@ def test_gemm_4096_sampled(gpu, dtype, layout, ta, tb):
             C0[sel_rows][:, sel_cols] if isinstance(sel_rows, np.ndarray) else C0[:, sel_cols])
         g = got[sel_rows][:, sel_cols] if isinstance(sel_rows, np.ndarray) else got[:, sel_cols]
         assert_gemm_close(g, want, mag, k, dtype, what="4096 sampled")
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("m,n,k", [(200, 190, 8192), (64, 48, 20000), (1024, 96, 4100)])
+def test_gemm_split_k_exact_and_deterministic(gpu, dtype, m, n, k):
+    """Shapes whose tile grid leaves most CUs idle run with k split over
+    several workgroups and an ordered reduction of the partial tiles: exact
+    on integers (every partial sum is exact), and repeatable bit for bit on
+    random data."""
+    rng = np.random.default_rng(m + n + k)
+    for ta, tb in ((0, 0), (1, 1), (0, 1)):
+        a, lda, b, ldb, c, ldc = make_problem(COL, ta, tb, m, n, k, 3, dtype, 2)
+        for x in (a, b, c):
+            x[:] = rng.integers(-3, 4, x.size)
+        c0 = c.copy()
+        level3.gemm(COL, ta, tb, m, n, k, 1.0, a, lda, b, ldb, 2.0, c, ldc)
+        want, _ = gemm_expect(COL, ta, tb, m, n, k, 1.0, a, lda, b, ldb, 2.0, c0, ldc)
+        assert np.array_equal(view(c, COL, m, n, ldc).astype(np.float64), want)
+    a, lda, b, ldb, c, ldc = make_problem(COL, 0, 1, m, n, k, 3, dtype, 4)
+    c1, c2 = c.copy(), c.copy()
+    level3.gemm(COL, 0, 1, m, n, k, 1.5, a, lda, b, ldb, -0.25, c1, ldc)
+    level3.gemm(COL, 0, 1, m, n, k, 1.5, a, lda, b, ldb, -0.25, c2, ldc)
+    assert np.array_equal(c1, c2)
+    want, mag = gemm_expect(COL, 0, 1, m, n, k, 1.5, a, lda, b, ldb, -0.25, c, ldc)
+    assert_gemm_close(view(c1, COL, m, n, ldc), want, mag, k, dtype, what="split-k")
