@@ -275,6 +275,81 @@ int32_t rblas_daxpy_dev(size_t n, double alpha, const double *x, size_t x_len,
                         size_t incx, double *y, size_t y_len, size_t incy,
                         void *stream);
 
+/* ------------------------------------------------------------ grouped gemm
+ * No rblas counterpart: rblas runs one `level3::gemm` (level3.rs:32-62) per
+ * call. A grouped plan holds many independent gemm problems, each with
+ * exactly rblas's gemm arguments and meaning (any Layout / Transpose, alpha,
+ * beta), on HBM-resident operands, and runs all of them in ONE persistent
+ * launch. Creating a plan validates every problem as rblas_sgemm_dev would
+ * (same RBLAS_PANIC / RBLAS_BAD_ENUM / RBLAS_BAD_ARG conditions; the first
+ * failing problem's index goes to *bad_index and no plan is made), chooses
+ * each problem's tile side and k splits from its shape alone (so a problem
+ * gives the same bits in any group), orders the work units longest first and
+ * uploads descriptors. Operand pointers are captured: a plan reruns on
+ * whatever those buffers hold at run time. C blocks of different problems
+ * must not overlap. One run of a plan at a time. */
+typedef struct rblas_grouped rblas_grouped_t;
+
+typedef struct {
+  int32_t layout, transa, transb;
+  size_t m, n, k;
+  float alpha;
+  const float *a;
+  size_t a_len, lda;
+  const float *b;
+  size_t b_len, ldb;
+  float beta;
+  float *c;
+  size_t c_len, ldc;
+} rblas_sgemm_problem_t;
+
+typedef struct {
+  int32_t layout, transa, transb;
+  size_t m, n, k;
+  double alpha;
+  const double *a;
+  size_t a_len, lda;
+  const double *b;
+  size_t b_len, ldb;
+  double beta;
+  double *c;
+  size_t c_len, ldc;
+} rblas_dgemm_problem_t;
+
+typedef struct {
+  size_t problems;        /* problems in the plan                            */
+  size_t units;           /* work units (C tile x k split) of the launch     */
+  int32_t grid;           /* workgroups of the persistent launch            */
+  int32_t tiles64;        /* 64 x 64 C tiles                                 */
+  int32_t tiles128;       /* 128 x 128 C tiles                               */
+  int32_t split_problems; /* problems whose k is split                      */
+  int32_t scale_problems; /* problems that only scale C (k == 0, alpha == 0) */
+  int32_t pad;
+  double flops;           /* 2 m n k summed over product problems            */
+  double est_us;          /* planner's cost-model estimate of one run        */
+  size_t workspace_bytes; /* device memory held by the plan                 */
+} rblas_grouped_info_t;
+
+int32_t rblas_sgemm_grouped_create(rblas_grouped_t **plan,
+                                   const rblas_sgemm_problem_t *problems,
+                                   size_t count, size_t *bad_index);
+int32_t rblas_dgemm_grouped_create(rblas_grouped_t **plan,
+                                   const rblas_dgemm_problem_t *problems,
+                                   size_t count, size_t *bad_index);
+/* Enqueue one run on `stream` (not synchronised). */
+int32_t rblas_grouped_run(rblas_grouped_t *plan, void *stream);
+/* The same run, captured into a hipGraph on first use and replayed after;
+ * needs a non-null stream. */
+int32_t rblas_grouped_run_graph(rblas_grouped_t *plan, void *stream);
+int32_t rblas_grouped_info(const rblas_grouped_t *plan, rblas_grouped_info_t *info);
+int32_t rblas_grouped_destroy(rblas_grouped_t *plan);
+/* The planner's cost model for dtype 0 (f32) or 1 (f64): coeffs =
+ * {c0_64, c1_64, c0_128, c1_128, unit_cap_us}: a unit of a tile side costs
+ * c0 + c1 * (its k / k-tile) microseconds; units above unit_cap_us get their k
+ * split. Written by the autotuner; affects plans created afterwards. */
+int32_t rblas_grouped_set_tuning(int32_t dtype, const double *coeffs);
+int32_t rblas_grouped_get_tuning(int32_t dtype, double *coeffs);
+
 /* ------------------------------------------------- device memory & streams
  * Plumbing for callers that keep operands resident in HBM (no rblas
  * counterpart). All return a status. */

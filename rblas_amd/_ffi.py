@@ -29,6 +29,31 @@ class Complex64(C.Structure):
     _fields_ = [("re", C.c_double), ("im", C.c_double)]
 
 
+def _problem_fields(t):
+    return [("layout", C.c_int32), ("transa", C.c_int32), ("transb", C.c_int32),
+            ("m", sz), ("n", sz), ("k", sz), ("alpha", t), ("a", vp), ("a_len", sz),
+            ("lda", sz), ("b", vp), ("b_len", sz), ("ldb", sz), ("beta", t), ("c", vp),
+            ("c_len", sz), ("ldc", sz)]
+
+
+class SgemmProblem(C.Structure):
+    """`rblas_sgemm_problem_t`: one gemm of a grouped plan (f32)."""
+    _fields_ = _problem_fields(C.c_float)
+
+
+class DgemmProblem(C.Structure):
+    """`rblas_dgemm_problem_t`: one gemm of a grouped plan (f64)."""
+    _fields_ = _problem_fields(C.c_double)
+
+
+class GroupedInfo(C.Structure):
+    """`rblas_grouped_info_t`."""
+    _fields_ = [("problems", sz), ("units", sz), ("grid", C.c_int32), ("tiles64", C.c_int32),
+                ("tiles128", C.c_int32), ("split_problems", C.c_int32),
+                ("scale_problems", C.c_int32), ("pad", C.c_int32), ("flops", C.c_double),
+                ("est_us", C.c_double), ("workspace_bytes", sz)]
+
+
 def _gemm_args(t):
     return [i32, i32, i32, sz, sz, sz, t, vp, sz, sz, vp, sz, sz, t, vp, sz, sz]
 
@@ -73,6 +98,16 @@ SIGNATURES: dict[str, tuple[list, object]] = {
     "rblas_dnrm2_dev": ([sz, vp, sz, sz, vp, vp], i32),
     "rblas_saxpy_dev": ([sz, f32, vp, sz, sz, vp, sz, sz, vp], i32),
     "rblas_daxpy_dev": ([sz, f64, vp, sz, sz, vp, sz, sz, vp], i32),
+    "rblas_sgemm_grouped_create": ([C.POINTER(vp), C.POINTER(SgemmProblem), sz,
+                                    C.POINTER(sz)], i32),
+    "rblas_dgemm_grouped_create": ([C.POINTER(vp), C.POINTER(DgemmProblem), sz,
+                                    C.POINTER(sz)], i32),
+    "rblas_grouped_run": ([vp, vp], i32),
+    "rblas_grouped_run_graph": ([vp, vp], i32),
+    "rblas_grouped_info": ([vp, C.POINTER(GroupedInfo)], i32),
+    "rblas_grouped_destroy": ([vp], i32),
+    "rblas_grouped_set_tuning": ([i32, C.POINTER(f64)], i32),
+    "rblas_grouped_get_tuning": ([i32, C.POINTER(f64)], i32),
     "rblas_malloc": ([C.POINTER(vp), sz], i32),
     "rblas_free": ([vp], i32),
     "rblas_memcpy_htod": ([vp, vp, sz, vp], i32),

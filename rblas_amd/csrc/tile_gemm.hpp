@@ -300,7 +300,13 @@ struct TileGemm {
 
   // Set up the tile at (m0, n0) over k range [kb, ke) of problem p.
   __device__ __forceinline__ void init(const GemmProblem<T> &p, int m0, int n0, int kb, int ke) {
-    tid = threadIdx.x;
+    // Opaque copy of the thread id: in a persistent loop over tiles every
+    // address derived from it is recomputed per tile instead of being hoisted
+    // out of the loop for every tile variant at once (which would keep them
+    // all live together).
+    int t = threadIdx.x;
+    asm volatile("" : "+v"(t));
+    tid = t;
     lane = tid & 63;
     const int wave = tid >> 6;
     wm = (wave & 1) * (BM / 2);
