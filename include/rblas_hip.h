@@ -336,6 +336,45 @@ int32_t rblas_sgemm_grouped_create(rblas_grouped_t **plan,
 int32_t rblas_dgemm_grouped_create(rblas_grouped_t **plan,
                                    const rblas_dgemm_problem_t *problems,
                                    size_t count, size_t *bad_index);
+/* ------------------------------------------------------------ grouped trsm
+ * No rblas counterpart: a plan of many independent `level3::trsm` problems
+ * (level3.rs:390-520), each with rblas's arguments and meaning, on
+ * HBM-resident operands. Creating it validates every problem as rblas_strsm
+ * would (first failing index to *bad_index), and additionally requires the
+ * order of A (m for Left, n for Right) to be at most 2048 (RBLAS_BAD_ARG
+ * above; rblas_?trsm_dev has no such limit). A run is one batched launch
+ * inverting every 64 x 64 diagonal block (f64), then one persistent launch
+ * in which a workgroup solves one system for 16 (order <= 1024) or 8
+ * right-hand sides held in LDS, in f64. B blocks of different problems must
+ * not overlap. Run, replay, info and destroy as for grouped gemm plans; info
+ * reports flops as the sum of m^2 n (Left) or m n^2 (Right). */
+typedef struct {
+  int32_t side, uplo, trans, diag;
+  size_t m, n;
+  float alpha;
+  const float *a;
+  size_t a_len, lda;
+  float *b;
+  size_t b_len, ldb;
+} rblas_strsm_problem_t;
+
+typedef struct {
+  int32_t side, uplo, trans, diag;
+  size_t m, n;
+  double alpha;
+  const double *a;
+  size_t a_len, lda;
+  double *b;
+  size_t b_len, ldb;
+} rblas_dtrsm_problem_t;
+
+int32_t rblas_strsm_grouped_create(rblas_grouped_t **plan,
+                                   const rblas_strsm_problem_t *problems,
+                                   size_t count, size_t *bad_index);
+int32_t rblas_dtrsm_grouped_create(rblas_grouped_t **plan,
+                                   const rblas_dtrsm_problem_t *problems,
+                                   size_t count, size_t *bad_index);
+
 /* Enqueue one run on `stream` (not synchronised). */
 int32_t rblas_grouped_run(rblas_grouped_t *plan, void *stream);
 /* The same run, captured into a hipGraph on first use and replayed after;

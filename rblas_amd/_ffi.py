@@ -46,6 +46,22 @@ class DgemmProblem(C.Structure):
     _fields_ = _problem_fields(C.c_double)
 
 
+def _trsm_problem_fields(t):
+    return [("side", C.c_int32), ("uplo", C.c_int32), ("trans", C.c_int32), ("diag", C.c_int32),
+            ("m", sz), ("n", sz), ("alpha", t), ("a", vp), ("a_len", sz), ("lda", sz),
+            ("b", vp), ("b_len", sz), ("ldb", sz)]
+
+
+class StrsmProblem(C.Structure):
+    """`rblas_strsm_problem_t`: one trsm of a grouped plan (f32)."""
+    _fields_ = _trsm_problem_fields(C.c_float)
+
+
+class DtrsmProblem(C.Structure):
+    """`rblas_dtrsm_problem_t`: one trsm of a grouped plan (f64)."""
+    _fields_ = _trsm_problem_fields(C.c_double)
+
+
 class GroupedInfo(C.Structure):
     """`rblas_grouped_info_t`."""
     _fields_ = [("problems", sz), ("units", sz), ("grid", C.c_int32), ("tiles64", C.c_int32),
@@ -101,6 +117,10 @@ SIGNATURES: dict[str, tuple[list, object]] = {
     "rblas_sgemm_grouped_create": ([C.POINTER(vp), C.POINTER(SgemmProblem), sz,
                                     C.POINTER(sz)], i32),
     "rblas_dgemm_grouped_create": ([C.POINTER(vp), C.POINTER(DgemmProblem), sz,
+                                    C.POINTER(sz)], i32),
+    "rblas_strsm_grouped_create": ([C.POINTER(vp), C.POINTER(StrsmProblem), sz,
+                                    C.POINTER(sz)], i32),
+    "rblas_dtrsm_grouped_create": ([C.POINTER(vp), C.POINTER(DtrsmProblem), sz,
                                     C.POINTER(sz)], i32),
     "rblas_grouped_run": ([vp, vp], i32),
     "rblas_grouped_run_graph": ([vp, vp], i32),

@@ -42,4 +42,36 @@ hipError_t launch_grouped_gemm(const GroupDesc<T> *descs, const WorkUnit *units,
 
 hipError_t launch_fill_ints(int *x, int64_t n, int v, hipStream_t stream);
 
+// One trsm problem of a group, in Left form: op(A) X = alpha B on `nrhs`
+// right-hand sides of length q. A Right problem (X op(A) = alpha B) is stored
+// as its transpose: trans and upper flipped, right = 1 (right-hand side c is
+// row c of B).
+template <typename T>
+struct TrsmDesc {
+  const T *a;
+  T *b;
+  int64_t lda, ldb;
+  int q, nrhs;
+  int trans, upper, unit, right;
+  int forward;  // substitution from row 0 up (lower triangle in Left form)
+  int width;    // right-hand sides per work unit: 16 (q <= 1024) or 8 (q <= 2048)
+  T alpha;
+  int64_t inv0;  // element offset of this problem's diagonal-block inverses
+};
+
+// q * width must fit the unit's LDS (f64).
+constexpr int kTrsmLdsDoubles = 16384;
+
+template <typename T>
+int grouped_trsm_grid();
+
+// Inverts the diagonal blocks listed in `blocks` ({problem, block}), then
+// runs one persistent launch over `units` ({problem, first right-hand
+// side}). sched[0..1] as for launch_grouped_gemm; `inv` holds 4096 f64 per
+// diagonal block.
+template <typename T>
+hipError_t launch_grouped_trsm(const TrsmDesc<T> *descs, const int2 *blocks, int nblocks,
+                               const int2 *units, int nunits, int grid, int *sched, double *inv,
+                               hipStream_t stream);
+
 }  // namespace rblas

@@ -10,14 +10,18 @@
 #include "gemm_host.hpp"
 #include "grouped.hpp"
 #include "runtime.hpp"
+#include "trsm_host.hpp"
 
 struct rblas_grouped {
+  int kind = 0;   // 0 gemm, 1 trsm
   int dtype = 0;  // 0 f32, 1 f64
   int device = 0;
-  void *mem = nullptr;  // descriptors | units | sched | counters | partials
+  void *mem = nullptr;  // descriptors | units | sched | counters (| blocks) | workspace
   void *descs = nullptr, *units = nullptr, *ws = nullptr;
   int *sched = nullptr, *counters = nullptr;
   int nunits = 0, grid = 0;
+  void *blocks = nullptr;  // trsm: diagonal blocks to invert
+  int nblocks = 0;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   rblas_grouped_info_t info{};
@@ -252,7 +256,129 @@ int32_t grouped_gemm_create(rblas_grouped_t **out, const P *probs, size_t count,
   return RBLAS_OK;
 }
 
+template <typename T, typename P>
+int32_t grouped_trsm_create(rblas_grouped_t **out, const P *probs, size_t count,
+                            size_t *bad_index) {
+  if (out) *out = nullptr;
+  if (bad_index) *bad_index = 0;
+  if (!out || (count && !probs)) return fail(bad_arg("grouped trsm: null plan or problem pointer"), nullptr);
+  if (count > (size_t)INT_MAX) return fail(bad_arg("grouped trsm: too many problems"), nullptr);
+  std::vector<TrsmDesc<T>> descs;
+  std::vector<int2> blocks, units;
+  std::vector<double> cost;
+  int64_t inv_elems = 0;
+  rblas_grouped_info_t info{};
+  info.problems = count;
+  for (size_t i = 0; i < count; ++i) {
+    const P &q = probs[i];
+    Check chk = trsm_validate<T>(q.side, q.uplo, q.trans, q.diag, q.m, q.n, q.a_len, q.lda,
+                                 q.b_len, q.ldb);
+    if (chk.ok() && q.m != 0 && q.n != 0 && (q.side == RBLAS_LEFT ? q.m : q.n) > 2048)
+      chk = bad_arg("grouped trsm: the order of A must be at most 2048 (use rblas_?trsm_dev)");
+    if (!chk.ok()) {
+      if (bad_index) *bad_index = i;
+      chk.msg = "problem " + std::to_string(i) + ": " + chk.msg;
+      return fail(chk, nullptr);
+    }
+    if (q.m == 0 || q.n == 0) continue;
+    const bool left = q.side == RBLAS_LEFT;
+    const bool t = is_transposed(q.trans);
+    const bool eff_upper = (q.uplo == RBLAS_UPPER) != t;
+    TrsmDesc<T> d{};
+    d.a = q.a;
+    d.b = q.b;
+    d.lda = (int64_t)q.lda;
+    d.ldb = (int64_t)q.ldb;
+    d.q = (int)(left ? q.m : q.n);
+    d.nrhs = (int)(left ? q.n : q.m);
+    // Left form: a Right problem solves op(A)^T X^T = alpha B^T.
+    d.trans = left ? t : !t;
+    d.upper = left ? eff_upper : !eff_upper;
+    d.unit = q.diag == RBLAS_UNIT;
+    d.right = !left;
+    d.forward = !d.upper;
+    d.width = d.q <= kTrsmLdsDoubles / 16 ? 16 : 8;
+    d.alpha = q.alpha;
+    d.inv0 = inv_elems;
+    const int prob = (int)descs.size();
+    const int nblk = (d.q + 63) / 64;
+    inv_elems += (int64_t)nblk * 64 * 64;
+    for (int b = 0; b < nblk; ++b) blocks.push_back(int2{prob, b});
+    for (int c0 = 0; c0 < d.nrhs; c0 += d.width) {
+      units.push_back(int2{prob, c0});
+      cost.push_back((double)d.q * (double)d.q * d.width);
+    }
+    info.flops += (double)d.q * (double)d.q * (double)d.nrhs;
+    descs.push_back(d);
+  }
+  std::vector<int> order(units.size());
+  for (size_t i = 0; i < order.size(); ++i) order[i] = (int)i;
+  std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return cost[x] > cost[y]; });
+  std::vector<int2> sorted(units.size());
+  for (size_t i = 0; i < order.size(); ++i) sorted[i] = units[order[i]];
+
+  int device = 0;
+  RBLAS_HIP_TRY(hipGetDevice(&device), "grouped trsm device");
+  rblas_grouped_t *g = new rblas_grouped_t();
+  g->kind = 1;
+  g->dtype = sizeof(T) == 4 ? 0 : 1;
+  g->device = device;
+  g->nunits = (int)sorted.size();
+  g->nblocks = (int)blocks.size();
+  g->grid = grouped_trsm_grid<T>();
+  const size_t b_desc = align256(descs.size() * sizeof(TrsmDesc<T>));
+  const size_t b_units = align256(sorted.size() * sizeof(int2));
+  const size_t b_sched = 256;
+  const size_t b_blocks = align256(blocks.size() * sizeof(int2));
+  const size_t b_inv = (size_t)inv_elems * sizeof(double);
+  const size_t total = b_desc + b_units + b_sched + b_blocks + b_inv;
+  hipError_t e = hipMalloc(&g->mem, total ? total : 256);
+  if (e != hipSuccess) {
+    delete g;
+    return hip_fail(e, "grouped trsm plan memory");
+  }
+  char *base = (char *)g->mem;
+  g->descs = base;
+  g->units = base + b_desc;
+  g->sched = (int *)(base + b_desc + b_units);
+  g->blocks = base + b_desc + b_units + b_sched;
+  g->ws = base + b_desc + b_units + b_sched + b_blocks;
+  e = hipMemcpy(g->descs, descs.data(), descs.size() * sizeof(TrsmDesc<T>), hipMemcpyHostToDevice);
+  if (e == hipSuccess)
+    e = hipMemcpy(g->units, sorted.data(), sorted.size() * sizeof(int2), hipMemcpyHostToDevice);
+  if (e == hipSuccess)
+    e = hipMemcpy(g->blocks, blocks.data(), blocks.size() * sizeof(int2), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemset(g->sched, 0, b_sched);
+  if (e == hipSuccess) e = hipDeviceSynchronize();
+  if (e != hipSuccess) {
+    hipFree(g->mem);
+    delete g;
+    return hip_fail(e, "grouped trsm plan upload");
+  }
+  info.units = sorted.size();
+  info.grid = std::min(g->grid, std::max(1, g->nunits));
+  info.workspace_bytes = total;
+  g->info = info;
+  *out = g;
+  return RBLAS_OK;
+}
+
 int32_t grouped_launch(rblas_grouped_t *g, hipStream_t st) {
+  if (g->kind == 1) {
+    if (g->dtype == 0)
+      RBLAS_HIP_TRY(launch_grouped_trsm<float>((const TrsmDesc<float> *)g->descs,
+                                               (const int2 *)g->blocks, g->nblocks,
+                                               (const int2 *)g->units, g->nunits, g->grid,
+                                               g->sched, (double *)g->ws, st),
+                    "grouped trsm launch");
+    else
+      RBLAS_HIP_TRY(launch_grouped_trsm<double>((const TrsmDesc<double> *)g->descs,
+                                                (const int2 *)g->blocks, g->nblocks,
+                                                (const int2 *)g->units, g->nunits, g->grid,
+                                                g->sched, (double *)g->ws, st),
+                    "grouped trsm launch");
+    return RBLAS_OK;
+  }
   if (g->dtype == 0)
     RBLAS_HIP_TRY(launch_grouped_gemm<float>((const GroupDesc<float> *)g->descs,
                                              (const WorkUnit *)g->units, g->nunits, g->grid,
@@ -289,6 +415,16 @@ int32_t rblas_sgemm_grouped_create(rblas_grouped_t **plan, const rblas_sgemm_pro
 int32_t rblas_dgemm_grouped_create(rblas_grouped_t **plan, const rblas_dgemm_problem_t *problems,
                                    size_t count, size_t *bad_index) {
   return grouped_gemm_create<double>(plan, problems, count, bad_index);
+}
+
+int32_t rblas_strsm_grouped_create(rblas_grouped_t **plan, const rblas_strsm_problem_t *problems,
+                                   size_t count, size_t *bad_index) {
+  return grouped_trsm_create<float>(plan, problems, count, bad_index);
+}
+
+int32_t rblas_dtrsm_grouped_create(rblas_grouped_t **plan, const rblas_dtrsm_problem_t *problems,
+                                   size_t count, size_t *bad_index) {
+  return grouped_trsm_create<double>(plan, problems, count, bad_index);
 }
 
 int32_t rblas_grouped_run(rblas_grouped_t *plan, void *stream) {
