@@ -389,6 +389,74 @@ int32_t rblas_grouped_destroy(rblas_grouped_t *plan);
 int32_t rblas_grouped_set_tuning(int32_t dtype, const double *coeffs);
 int32_t rblas_grouped_get_tuning(int32_t dtype, double *coeffs);
 
+/* ----------------------------------------------------------------- serving
+ * No rblas counterpart. Replays a request stream against the wall clock on
+ * the device: request i arrives arrival_s seconds after the start (arrivals
+ * non-decreasing), joins a FIFO queue, is admitted when a stream slot is free
+ * and the footprint of all admitted, unfinished requests stays within
+ * hbm_cap_bytes, and completes when its batch's work has finished. Operands
+ * are synthetic, generated in HBM from `seed` at launch (workloads.py's
+ * counter-based splitmix64, draw order A, B, C; uniform(-1, 1); a trsm A is
+ * lower triangular: off-diagonal u / sqrt(m), diagonal 2 + |u|) and freed on
+ * completion. Kinds (column-major, NoTrans):
+ *   RBLAS_SERVE_GEMM   C (m x n) := 1 * A (m x k) B (k x n) + 0.5 C, dtype 0 f32 / 1 f64
+ *   RBLAS_SERVE_TRSM   A (m x m) X = B (m x n), Left, Lower, NonUnit, alpha 1
+ *   RBLAS_SERVE_ZGEMM  complex f64: C := (1 + 0i) A B + (0.5 + 0i) C (4M)
+ * Mode RBLAS_SERVE_BATCHED: continuous batching over max_inflight streams; a
+ * batch is the queue's admitted head (at most max_batch requests): one
+ * grouped gemm launch per dtype, one grouped trsm launch, 4M launches per
+ * complex request. Mode RBLAS_SERVE_PLAIN: the per-call comparator, one
+ * request at a time in FIFO order through the single-call kernels.
+ * results[i] gets the request's times (seconds from the start), algorithmic
+ * flops (2mnk, m^2 n, 8mnk), footprint and an order-independent 64-bit hash
+ * of the result's bits; a request's hash does not depend on what else ran
+ * (same batch mode). capture_ids / capture_out (optional, tests): the dense
+ * result of those requests is copied to the given host buffers. Blocks until
+ * every request has completed. */
+enum { RBLAS_SERVE_GEMM = 0, RBLAS_SERVE_TRSM = 1, RBLAS_SERVE_ZGEMM = 2 };
+enum { RBLAS_SERVE_BATCHED = 0, RBLAS_SERVE_PLAIN = 1 };
+
+typedef struct {
+  int32_t kind;
+  int32_t dtype;    /* 0 f32, 1 f64 (complex requests are f64) */
+  int32_t m, n, k;  /* trsm: m = order of A, n = right-hand sides, k unused */
+  int32_t pad;
+  uint64_t seed;
+  double arrival_s;
+} rblas_serve_request_t;
+
+typedef struct {
+  int32_t mode;
+  int32_t max_inflight; /* batches on the device at once (batched mode)      */
+  int32_t max_batch;    /* requests per batch (batched mode)                 */
+  int32_t capture_count;
+  double hbm_cap_bytes; /* admission cap on admitted, unfinished footprints  */
+  const int32_t *capture_ids;
+  void *const *capture_out;
+} rblas_serve_config_t;
+
+typedef struct {
+  double arrival_s, admit_s, done_s;
+  double flops;
+  double footprint_bytes;
+  uint64_t hash;
+  int32_t batch;
+  int32_t pad;
+} rblas_serve_result_t;
+
+typedef struct {
+  double wall_s;              /* start to last completion                   */
+  double flops;               /* sum of the requests' algorithmic flops     */
+  double p50_s, p99_s;        /* latency (completion - arrival) percentiles */
+  double peak_footprint_bytes;
+  int32_t batches;
+  int32_t max_inflight_seen;
+} rblas_serve_stats_t;
+
+int32_t rblas_serve_run(const rblas_serve_request_t *requests, size_t count,
+                        const rblas_serve_config_t *config,
+                        rblas_serve_result_t *results, rblas_serve_stats_t *stats);
+
 /* ------------------------------------------------- device memory & streams
  * Plumbing for callers that keep operands resident in HBM (no rblas
  * counterpart). All return a status. */

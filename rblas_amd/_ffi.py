@@ -70,6 +70,20 @@ class GroupedInfo(C.Structure):
                 ("est_us", C.c_double), ("workspace_bytes", sz)]
 
 
+class ServeConfig(C.Structure):
+    """`rblas_serve_config_t`."""
+    _fields_ = [("mode", C.c_int32), ("max_inflight", C.c_int32), ("max_batch", C.c_int32),
+                ("capture_count", C.c_int32), ("hbm_cap_bytes", C.c_double),
+                ("capture_ids", C.POINTER(C.c_int32)), ("capture_out", C.POINTER(C.c_void_p))]
+
+
+class ServeStats(C.Structure):
+    """`rblas_serve_stats_t`."""
+    _fields_ = [("wall_s", C.c_double), ("flops", C.c_double), ("p50_s", C.c_double),
+                ("p99_s", C.c_double), ("peak_footprint_bytes", C.c_double),
+                ("batches", C.c_int32), ("max_inflight_seen", C.c_int32)]
+
+
 def _gemm_args(t):
     return [i32, i32, i32, sz, sz, sz, t, vp, sz, sz, vp, sz, sz, t, vp, sz, sz]
 
@@ -128,6 +142,7 @@ SIGNATURES: dict[str, tuple[list, object]] = {
     "rblas_grouped_destroy": ([vp], i32),
     "rblas_grouped_set_tuning": ([i32, C.POINTER(f64)], i32),
     "rblas_grouped_get_tuning": ([i32, C.POINTER(f64)], i32),
+    "rblas_serve_run": ([vp, sz, C.POINTER(ServeConfig), vp, C.POINTER(ServeStats)], i32),
     "rblas_malloc": ([C.POINTER(vp), sz], i32),
     "rblas_free": ([vp], i32),
     "rblas_memcpy_htod": ([vp, vp, sz, vp], i32),

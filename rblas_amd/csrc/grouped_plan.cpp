@@ -1,0 +1,232 @@
+// Grouped planner: per-problem launch shape from the measured cost model,
+// work units ordered longest first, and the staged device tables.
+#include "grouped_plan.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <mutex>
+
+namespace rblas {
+namespace {
+
+// Defaults: the autotuner's measurement on MI355X (rblas_amd/autotune.py,
+// saved in rblas_amd/tuning_gfx950.json). The unit cap sits just above where
+// splitting stopped paying on the batching workload, so only very long-k
+// units are split.
+Tuning g_tuning = {{{6.582, 16.869}, {6.517, 17.415}},
+                   {{0.791, 2.306}, {0.737, 2.136}},
+                   {320.0, 320.0}};
+std::mutex g_mu;
+
+constexpr int kMinSplitKtiles = 16;
+constexpr int kMaxSplits = 16;
+
+int k_tile(int dtype) { return dtype == 0 ? 32 : 16; }
+
+struct Shape {
+  int tile, splits, tiles_m, tiles;
+  double unit_us;
+};
+
+// The launch shape of one product problem: a function of (dtype, m, n, k)
+// and the tuning table only, so a problem gives the same bits in any group.
+Shape plan_shape(const Tuning &tu, int dtype, int64_t m, int64_t n, int64_t k) {
+  const int64_t kt = (k + k_tile(dtype) - 1) / k_tile(dtype);
+  Shape best{};
+  double best_total = 0;
+  for (int ti = 1; ti >= 0; --ti) {  // 128 first: ties keep the bigger tile
+    const int tile = ti ? 128 : 64;
+    const int64_t tm = (m + tile - 1) / tile, tn = (n + tile - 1) / tile;
+    const double unit = tu.c0[dtype][ti] + tu.c1[dtype][ti] * (double)kt;
+    const double total = (double)(tm * tn) * unit;
+    if (ti == 1 || total < best_total) {
+      best = Shape{tile, 1, (int)tm, (int)(tm * tn), unit};
+      best_total = total;
+    }
+  }
+  const int ti = best.tile == 128 ? 1 : 0;
+  if (best.unit_us > tu.unit_cap_us[dtype]) {
+    int64_t s = (int64_t)std::ceil(best.unit_us / tu.unit_cap_us[dtype]);
+    s = std::min<int64_t>(s, kt / kMinSplitKtiles);
+    s = std::min<int64_t>(s, kMaxSplits);
+    if (s >= 2) {
+      const int64_t per = (kt + s - 1) / s;
+      best.splits = (int)s;
+      best.unit_us = tu.c0[dtype][ti] + tu.c1[dtype][ti] * (double)per;
+    }
+  }
+  return best;
+}
+
+template <typename U>
+void order_longest_first(std::vector<U> &units, const std::vector<double> &cost) {
+  std::vector<int> order(units.size());
+  for (size_t i = 0; i < order.size(); ++i) order[i] = (int)i;
+  std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return cost[x] > cost[y]; });
+  std::vector<U> sorted(units.size());
+  for (size_t i = 0; i < order.size(); ++i) sorted[i] = units[order[i]];
+  units.swap(sorted);
+}
+
+}  // namespace
+
+Tuning tuning_snapshot() {
+  std::lock_guard<std::mutex> g(g_mu);
+  return g_tuning;
+}
+
+void tuning_set(int dtype, const double *c) {
+  std::lock_guard<std::mutex> g(g_mu);
+  g_tuning.c0[dtype][0] = c[0];
+  g_tuning.c1[dtype][0] = c[1];
+  g_tuning.c0[dtype][1] = c[2];
+  g_tuning.c1[dtype][1] = c[3];
+  g_tuning.unit_cap_us[dtype] = c[4];
+}
+
+void tuning_get(int dtype, double *c) {
+  std::lock_guard<std::mutex> g(g_mu);
+  c[0] = g_tuning.c0[dtype][0];
+  c[1] = g_tuning.c1[dtype][0];
+  c[2] = g_tuning.c0[dtype][1];
+  c[3] = g_tuning.c1[dtype][1];
+  c[4] = g_tuning.unit_cap_us[dtype];
+}
+
+template <typename T>
+void GemmGroup<T>::add(const GemmProblem<T> &p, const Tuning &tu) {
+  const int dtype = sizeof(T) == 4 ? 0 : 1;
+  GroupDesc<T> d{};
+  d.a = p.a;
+  d.b = p.b;
+  d.c = p.c;
+  d.lda = p.lda;
+  d.ldb = p.ldb;
+  d.ldc = p.ldc;
+  d.m = (int)p.m;
+  d.n = (int)p.n;
+  d.k = (int)p.k;
+  d.ta = p.ta;
+  d.tb = p.tb;
+  d.alpha = p.alpha;
+  d.beta = p.beta;
+  d.splits = 1;
+  const int prob = (int)descs.size();
+  if (p.k == 0 || p.alpha == T(0)) {
+    d.mode = 1;
+    d.tile = 128;
+    d.tiles_m = (int)((p.m + 127) / 128);
+    const int tiles = d.tiles_m * (int)((p.n + 127) / 128);
+    for (int t = 0; t < tiles; ++t) {
+      units.push_back(WorkUnit{prob, t, 0, 0});
+      cost.push_back(tu.c0[dtype][1]);
+      total_us += tu.c0[dtype][1];
+    }
+    ++info.scale_problems;
+    descs.push_back(d);
+    return;
+  }
+  const Shape sh = plan_shape(tu, dtype, p.m, p.n, p.k);
+  d.mode = 0;
+  d.tile = sh.tile;
+  d.splits = sh.splits;
+  d.tiles_m = sh.tiles_m;
+  if (sh.splits > 1) {
+    d.slot0 = ws_elems;
+    d.cnt0 = (int)counters;
+    ws_elems += (int64_t)sh.tiles * sh.splits * sh.tile * sh.tile;
+    counters += sh.tiles;
+    ++info.split_problems;
+  }
+  (sh.tile == 128 ? info.tiles128 : info.tiles64) += sh.tiles;
+  for (int t = 0; t < sh.tiles; ++t)
+    for (int s = 0; s < sh.splits; ++s) {
+      units.push_back(WorkUnit{prob, t, s, 0});
+      cost.push_back(sh.unit_us);
+      total_us += sh.unit_us;
+    }
+  info.flops += 2.0 * (double)p.m * (double)p.n * (double)p.k;
+  descs.push_back(d);
+}
+
+template <typename T>
+void GemmGroup<T>::finish() {
+  order_longest_first(units, cost);
+  info.units = units.size();
+}
+
+template <typename T>
+void GemmGroup<T>::stage(char *h) const {
+  std::memset(h, 0, staged_bytes());
+  std::memcpy(h, descs.data(), descs.size() * sizeof(GroupDesc<T>));
+  std::memcpy(h + off_units(), units.data(), units.size() * sizeof(WorkUnit));
+}
+
+template <typename T>
+hipError_t GemmGroup<T>::launch(char *dev, int grid, hipStream_t st, char *ws) const {
+  return launch_grouped_gemm<T>((const GroupDesc<T> *)dev, (const WorkUnit *)(dev + off_units()),
+                                (int)units.size(), grid, (int *)(dev + off_sched()),
+                                (T *)(ws ? ws : dev + off_ws()), (int *)(dev + off_sched() + 256),
+                                st);
+}
+
+template <typename T>
+void TrsmGroup<T>::add(bool left, bool trans, bool eff_upper, bool unit, int64_t m, int64_t n,
+                       T alpha, const T *a, int64_t lda, T *b, int64_t ldb) {
+  TrsmDesc<T> d{};
+  d.a = a;
+  d.b = b;
+  d.lda = lda;
+  d.ldb = ldb;
+  d.q = (int)(left ? m : n);
+  d.nrhs = (int)(left ? n : m);
+  // Left form: a Right problem solves op(A)^T X^T = alpha B^T.
+  d.trans = left ? trans : !trans;
+  d.upper = left ? eff_upper : !eff_upper;
+  d.unit = unit;
+  d.right = !left;
+  d.forward = !d.upper;
+  d.width = d.q <= kTrsmLdsDoubles / 16 ? 16 : 8;
+  d.alpha = alpha;
+  d.inv0 = inv_elems;
+  const int prob = (int)descs.size();
+  const int nblk = (d.q + 63) / 64;
+  inv_elems += (int64_t)nblk * 64 * 64;
+  for (int blk = 0; blk < nblk; ++blk) blocks.push_back(int2{prob, blk});
+  for (int c0 = 0; c0 < d.nrhs; c0 += d.width) {
+    units.push_back(int2{prob, c0});
+    cost.push_back((double)d.q * (double)d.q * d.width);
+  }
+  info.flops += (double)d.q * (double)d.q * (double)d.nrhs;
+  descs.push_back(d);
+}
+
+template <typename T>
+void TrsmGroup<T>::finish() {
+  order_longest_first(units, cost);
+  info.units = units.size();
+}
+
+template <typename T>
+void TrsmGroup<T>::stage(char *h) const {
+  std::memset(h, 0, staged_bytes());
+  std::memcpy(h, descs.data(), descs.size() * sizeof(TrsmDesc<T>));
+  std::memcpy(h + off_units(), units.data(), units.size() * sizeof(int2));
+  std::memcpy(h + off_blocks(), blocks.data(), blocks.size() * sizeof(int2));
+}
+
+template <typename T>
+hipError_t TrsmGroup<T>::launch(char *dev, int grid, hipStream_t st, char *ws) const {
+  return launch_grouped_trsm<T>((const TrsmDesc<T> *)dev, (const int2 *)(dev + off_blocks()),
+                                (int)blocks.size(), (const int2 *)(dev + off_units()),
+                                (int)units.size(), grid, (int *)(dev + off_sched()),
+                                (double *)(ws ? ws : dev + off_ws()), st);
+}
+
+template struct GemmGroup<float>;
+template struct GemmGroup<double>;
+template struct TrsmGroup<float>;
+template struct TrsmGroup<double>;
+
+}  // namespace rblas
