@@ -1,17 +1,32 @@
 #!/usr/bin/env python3
-"""Benchmark: DGEMM 8192^3 on one MI355X through librblas_hip.so.
+"""Benchmark of the MI355X Level-3 path through librblas_hip.so (one GPU).
 
-Workload (one "step" = one gemm call on HBM-resident operands):
-  C := 1.5 * A * B - 0.25 * C, f64, column-major NoTrans/NoTrans,
-  m = n = k = 8192, lda = ldb = ldc = 8199 (rows + 7), entries N(0,1) from
-  the seeded splitmix64 generator (rblas_amd.workloads), seed 1, drawn in
-  the order A, B, C.
+Headline (`value`): DGEMM 8192^3 TFLOP/s. One step = one gemm call on
+HBM-resident operands: C := 1.5 A B - 0.25 C, f64, column-major NN,
+m = n = k = 8192, lda = ldb = ldc = 8199 (rows + 7), entries N(0,1) from the
+seeded splitmix64 generator (rblas_amd.workloads), seed 1, drawn A, B, C.
+Timed with HIP events on the stream the kernel runs on.
 
-Prints one JSON line. `value` is whole-job TFLOP/s (2mnk flop per call) over
-the K timed calls, timed with HIP events on the stream the kernel runs on.
-`roofline` prices the gemm kernel against MI355X's FP64 MFMA peak;
-`cpu_baseline` times the oracle (rblas's CPU algorithm restated in C++) on a
-column slice of the same problem on the host cores.
+`components` measures the rest of the metric (and of the workloads), each on
+synthetic data of the stated shape:
+  - SGEMM 8192^3 in plain FP32, BF16x3 and BF16x6 (TFLOP/s, % of peak), and
+    the precision study: inputs (a) N(0,1), (b) rows of A scaled by
+    2^U[-20,20], (c) m = n = 1024, k = 65536; max |C - C64| / (|A||B|)
+    over sampled rows against a float64 product;
+  - grouped GEMM (512 problems, m, n, k log-uniform in [16, 2048], random
+    transposes, U[-1,1], f32 and f64) in one persistent launch, against the
+    same problems as one launch each;
+  - grouped trsm (256 lower systems, n log-uniform [32, 1024], nrhs in
+    [1, 256], diag 2+|N(0,1)|, off-diagonal N(0,1)/sqrt(n), f64) against
+    per-call trsm;
+  - serving: 10k requests (60% gemm, 25% trsm, 15% complex gemm), Poisson
+    at 70% of the measured closed-loop capacity, 200 GB admission cap:
+    p50/p99 latency and TFLOP/s, continuous batching vs plain per-call;
+  - C64 gemm (4M) 4096^3 and Level-1 bandwidth.
+`roofline` prices the DGEMM kernel against MI355X's FP64 MFMA peak, with HBM
+traffic per launch from the committed rocprofv3 PMC summary; `cpu_baseline`
+times the oracle (rblas's CPU algorithm restated in C++) on a column slice
+of the DGEMM problem on the host cores.
 """
 from __future__ import annotations
 
@@ -26,8 +41,10 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-FP64_MFMA_PEAK_TFLOPS = 78.6   # MI355X dense FP64 matrix
-FP32_MFMA_PEAK_TFLOPS = 157.3  # MI355X dense FP32 matrix
+FP64_PEAK = 78.6    # MI355X dense FP64 matrix TFLOP/s
+FP32_PEAK = 157.3   # MI355X dense FP32 matrix TFLOP/s
+HBM_PEAK_GBS = 8000.0
+METRIC = ("TFLOP/s & %peak: DGEMM, 3xBF16 SGEMM 8192^3; grouped GEMM TFLOP/s; serve p99 ms")
 
 
 def parse():
@@ -36,9 +53,9 @@ def parse():
     p.add_argument("--steps", type=int, default=10)
     p.add_argument("--warmup", type=int, default=3)
     p.add_argument("--size", type=int, default=8192)
+    p.add_argument("--quick", action="store_true", help="headline DGEMM only")
     p.add_argument("--no-cpu-baseline", action="store_true")
-    p.add_argument("--extra", action="store_true",
-                   help="also time SGEMM (plain FP32) at the same size")
+    p.add_argument("--serve-requests", type=int, default=10000)
     p.add_argument("--copy-calibrate", action="store_true",
                    help="only run 5 device-to-device copies of 1 GiB (counter calibration)")
     return p.parse_args()
@@ -47,8 +64,12 @@ def parse():
 PROFILE = os.path.join(ROOT, "profiles", "dgemm_8192_nn.md")
 
 
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
 def measured_traffic():
-    """HBM bytes per gemm launch from the committed rocprofv3 PMC summary
+    """HBM bytes per DGEMM launch from the committed rocprofv3 PMC summary
     (FETCH_SIZE + WRITE_SIZE, scaled by the copy calibration), or None."""
     try:
         for line in open(PROFILE):
@@ -71,37 +92,241 @@ def copy_calibrate():
     print(json.dumps({"copy_calibration_bytes": n, "copies": 5}), flush=True)
 
 
-def gen_operands(n, ld, dtype, seed=1):
-    from rblas_amd import workloads
-    la = (n - 1) * ld + n
-    a = workloads.normal(seed, 0, la, dtype)
-    b = workloads.normal(seed, la, la, dtype)
-    c = workloads.normal(seed, 2 * la, la, dtype)
-    return a, b, c
-
-
-def time_gemm(dtype, n, ld, steps, warmup):
-    from rblas_amd import level3
-    from rblas_amd.device import DeviceArray, Event, Stream
-    a, b, c = gen_operands(n, ld, dtype)
-    s = Stream()
-    da, db, dc = (DeviceArray.from_host(x, s) for x in (a, b, c))
-    del a, b, c
-    args = (1, 0, 0, n, n, n, 1.5, da, ld, db, ld, -0.25, dc, ld, s)
+def timed(fn, stream, steps, warmup):
+    from rblas_amd.device import Event
     for _ in range(warmup):
-        level3.gemm_dev(*args)
-    s.synchronize()
+        fn()
+    stream.synchronize()
     e0, e1 = Event(), Event()
-    e0.record(s)
+    e0.record(stream)
     for _ in range(steps):
-        level3.gemm_dev(*args)
-    e1.record(s)
-    s.synchronize()
-    ms = e1.elapsed_ms(e0)
+        fn()
+    e1.record(stream)
+    stream.synchronize()
+    ms = e1.elapsed_ms(e0) / steps
+    e0.close()
+    e1.close()
+    return ms
+
+
+def square_gemm(dtype, n, ld, steps, warmup, compute=None, operands=None):
+    """ms per call of the 8192-style gemm (optionally a split-SGEMM compute
+    mode); returns (ms, device C downloaded or None)."""
+    from rblas_amd import level3, workloads
+    from rblas_amd.device import DeviceArray, Stream
+    la = (n - 1) * ld + n
+    if operands is None:
+        operands = (workloads.normal(1, 0, la, dtype), workloads.normal(1, la, la, dtype),
+                    workloads.normal(1, 2 * la, la, dtype))
+    s = Stream()
+    da, db, dc = (DeviceArray.from_host(x, s) for x in operands)
+    if compute is None:
+        fn = lambda: level3.gemm_dev(1, 0, 0, n, n, n, 1.5, da, ld, db, ld, -0.25, dc, ld, s)
+    else:
+        fn = lambda: level3.sgemm_ex_dev(1, 0, 0, n, n, n, 1.5, da, ld, db, ld, -0.25, dc, ld,
+                                         compute, s)
+    ms = timed(fn, s, steps, warmup)
     for x in (da, db, dc):
         x.free()
     s.close()
-    return ms / steps
+    return ms
+
+
+def precision_case(case, rng_seed=3):
+    """Max |C - C64| / (|A||B|) over 48 sampled rows, for FP32, BF16x3 and
+    BF16x6 (alpha 1, beta 0), with the float64 product of the same rows as
+    the reference."""
+    from rblas_amd import level3, workloads
+    from rblas_amd.device import DeviceArray, Stream
+    from rblas_amd.types import Compute
+    if case == "c":
+        m = n = 1024
+        k = 65536
+    else:
+        m = n = k = 8192
+    A = workloads.normal(11, 0, m * k, np.float32)
+    B = workloads.normal(11, m * k, k * n, np.float32)
+    if case == "b":
+        rng = np.random.default_rng(rng_seed)
+        scale = np.exp2(rng.uniform(-20, 20, m)).astype(np.float32)
+        A = (A.reshape(k, m) * scale[None, :]).reshape(-1)  # row i of A scaled
+    rows = np.random.default_rng(5).choice(m, 48, replace=False)
+    Am = A.reshape(k, m).T[rows].astype(np.float64)
+    Bm = B.reshape(n, k).T.astype(np.float64)
+    ref = Am @ Bm
+    mag = np.abs(Am) @ np.abs(Bm)
+    s = Stream()
+    da, db = DeviceArray.from_host(A, s), DeviceArray.from_host(B, s)
+    dc = DeviceArray(m * n, np.float32)
+    out = {}
+    for name, comp in (("fp32", Compute.F32), ("bf16x3", Compute.BF16X3),
+                       ("bf16x6", Compute.BF16X6)):
+        level3.sgemm_ex_dev(1, 0, 0, m, n, k, 1.0, da, m, db, k, 0.0, dc, m, comp, s)
+        s.synchronize()
+        C = dc.download(s).reshape(n, m).T[rows].astype(np.float64)
+        out[name] = float((np.abs(C - ref) / mag).max())
+    for x in (da, db, dc):
+        x.free()
+    s.close()
+    out["bf16x3_over_fp32"] = out["bf16x3"] / out["fp32"]
+    out["bf16x6_over_fp32"] = out["bf16x6"] / out["fp32"]
+    return {k: float(f"{v:.4g}") for k, v in out.items()}
+
+
+def skinny_sgemm_tflops(steps=5):
+    """m = n = 1024, k = 65536 SGEMM (the precision study's case c)."""
+    from rblas_amd import level3, workloads
+    from rblas_amd.device import DeviceArray, Stream
+    from rblas_amd.types import Compute
+    m = n = 1024
+    k = 65536
+    s = Stream()
+    da = DeviceArray.from_host(workloads.uniform(2, 0, m * k, np.float32), s)
+    db = DeviceArray.from_host(workloads.uniform(2, m * k, k * n, np.float32), s)
+    dc = DeviceArray(m * n, np.float32)
+    out = {}
+    for name, comp in (("fp32", Compute.F32), ("bf16x3", Compute.BF16X3),
+                       ("bf16x6", Compute.BF16X6)):
+        ms = timed(lambda: level3.sgemm_ex_dev(1, 0, 0, m, n, k, 1.0, da, m, db, k, 0.0, dc, m,
+                                               comp, s), s, steps, 1)
+        out[name] = round(2.0 * m * n * k / ms / 1e9, 2)
+    for x in (da, db, dc):
+        x.free()
+    s.close()
+    return out
+
+
+def grouped_gemm(dtype):
+    from rblas_amd import grouped, level3, workloads
+    from rblas_amd.device import DeviceArray, Stream
+    rng = np.random.default_rng(4)
+    s = Stream()
+    probs, off, flops = [], 0, 0.0
+    for _ in range(512):
+        m, n, k = (int(np.exp(rng.uniform(np.log(16), np.log(2049)))) for _ in range(3))
+        ta, tb = int(rng.integers(0, 2)), int(rng.integers(0, 2))
+        ar, ac = (k, m) if ta else (m, k)
+        br, bc = (n, k) if tb else (k, n)
+        arrs = []
+        for cnt in (ar * ac, br * bc, m * n):
+            arrs.append(DeviceArray.from_host(workloads.uniform(21, off, cnt, dtype), s))
+            off += cnt
+        probs.append(grouped.GemmProblem(1, ta, tb, m, n, k, 1.0, arrs[0], ar, arrs[1], br, 0.5,
+                                         arrs[2], m))
+        flops += 2.0 * m * n * k
+    plan = grouped.GroupedGemm(probs, dtype)
+    ms = timed(lambda: plan.run(s), s, 10, 2)
+    ms_graph = timed(lambda: plan.run_graph(s), s, 10, 2)
+
+    def percall():
+        for p in probs:
+            level3.gemm_dev(1, p.transa, p.transb, p.m, p.n, p.k, 1.0, p.a, p.lda, p.b, p.ldb,
+                            0.5, p.c, p.ldc, s)
+    ms_pc = timed(percall, s, 3, 1)
+    info = plan.info()
+    peak = FP32_PEAK if dtype == np.float32 else FP64_PEAK
+    # Per-problem roofline: max(flops / peak, bytes / HBM) summed over problems.
+    sz = np.dtype(dtype).itemsize
+    ideal_s = 0.0
+    for p in probs:
+        f = 2.0 * p.m * p.n * p.k
+        b = (p.m * p.k + p.k * p.n + 2 * p.m * p.n) * sz
+        ideal_s += max(f / (peak * 1e12), b / (HBM_PEAK_GBS * 1e9))
+    plan.close()
+    for p in probs:
+        for x in (p.a, p.b, p.c):
+            x.free()
+    s.close()
+    return {"tflops": round(flops / ms / 1e9, 2), "ms": round(ms, 3),
+            "graph_ms": round(ms_graph, 3), "per_call_ms": round(ms_pc, 3),
+            "speedup_vs_per_call": round(ms_pc / ms, 2),
+            "roofline_frac": round(ideal_s * 1e3 / ms, 3), "units": info["units"],
+            "tiles64": info["tiles64"], "tiles128": info["tiles128"],
+            "split_problems": info["split_problems"]}
+
+
+def grouped_trsm():
+    from rblas_amd import grouped, level3, workloads
+    from rblas_amd.device import DeviceArray, Stream
+    rng = np.random.default_rng(5)
+    s = Stream()
+    probs, flops = [], 0.0
+    for _ in range(256):
+        q = int(np.exp(rng.uniform(np.log(32), np.log(1025))))
+        nr = int(rng.integers(1, 257))
+        full = rng.standard_normal((q, q)) / np.sqrt(q)
+        np.fill_diagonal(full, 2.0 + np.abs(rng.standard_normal(q)))
+        a = DeviceArray.from_host(np.tril(full).T.reshape(-1).copy(), s)
+        b = DeviceArray.from_host(rng.standard_normal(q * nr), s)
+        probs.append(grouped.TrsmProblem(0, 1, 0, 0, q, nr, 1.0, a, q, b, q))
+        flops += float(q) * q * nr
+    plan = grouped.GroupedTrsm(probs, np.float64)
+    ms = timed(lambda: plan.run(s), s, 10, 2)
+
+    def percall():
+        for p in probs:
+            level3.trsm_dev(0, 1, 0, 0, p.m, p.n, 1.0, p.a, p.lda, p.b, p.ldb, s)
+    ms_pc = timed(percall, s, 2, 1)
+    plan.close()
+    for p in probs:
+        p.a.free()
+        p.b.free()
+    s.close()
+    return {"tflops": round(flops / ms / 1e9, 3), "ms": round(ms, 3),
+            "per_call_ms": round(ms_pc, 3), "speedup_vs_per_call": round(ms_pc / ms, 1)}
+
+
+def serving(count):
+    from rblas_amd import serve
+    closed = serve.workload(count, 0.0)
+    res, st = serve.run(closed, serve.BATCHED)
+    capacity = count / st["wall_s"]
+    rate = 0.7 * capacity
+    trace = serve.workload(count, rate)
+    res, st = serve.run(trace, serve.BATCHED)
+    batched = serve.summary(res, st)
+    res2, st2 = serve.run(trace, serve.PLAIN)
+    plain = serve.summary(res2, st2)
+    keys = ("tflops", "p50_ms", "p99_ms", "wall_s", "batches", "peak_footprint_gb")
+    return {"requests": count, "rate_rps": round(rate, 1),
+            "capacity_rps": round(capacity, 1),
+            "batched": {k: round(batched[k], 3) for k in keys},
+            "plain": {k: round(plain[k], 3) for k in keys}}
+
+
+def c64_gemm(n=4096):
+    from rblas_amd import complex3, workloads
+    from rblas_amd.device import DeviceArray, Stream
+    s = Stream()
+    arrs = []
+    for i in range(3):
+        x = workloads.normal(13, 2 * i * n * n, 2 * n * n, np.float64).view(np.complex128)
+        arrs.append(DeviceArray.from_host(x, s))
+    ms = timed(lambda: complex3.gemm_dev(0, 0, n, n, n, 1.5 - 0.5j, arrs[0], n, arrs[1], n,
+                                         -0.25, arrs[2], n, s), s, 5, 1)
+    for x in arrs:
+        x.free()
+    s.close()
+    return {"tflops": round(8.0 * n ** 3 / ms / 1e9, 2), "ms": round(ms, 3)}
+
+
+def level1_bw(n=1 << 26):
+    from rblas_amd import level1, workloads
+    from rblas_amd.device import DeviceArray, Stream
+    s = Stream()
+    x = DeviceArray.from_host(workloads.uniform(3, 0, n, np.float64), s)
+    y = DeviceArray.from_host(workloads.uniform(3, n, n, np.float64), s)
+    r = DeviceArray(1, np.float64)
+    out = {}
+    for name, fn, nbytes in (("ddot", lambda: level1.dot_dev(n, x, 1, y, 1, r, s), 16 * n),
+                             ("daxpy", lambda: level1.axpy_dev(n, 1e-3, x, 1, y, 1, s), 24 * n),
+                             ("dnrm2", lambda: level1.nrm2_dev(n, x, 1, r, s), 8 * n)):
+        ms = timed(fn, s, 20, 3)
+        out[name + "_gbs"] = round(nbytes / ms / 1e6, 1)
+    for a in (x, y, r):
+        a.free()
+    s.close()
+    return out
 
 
 def cpu_baseline(n, ld):
@@ -138,13 +363,16 @@ def main():
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
                           "(multi-GPU sharding is described in DESIGN.md, not built)"}))
         sys.exit(2)
+    from rblas_amd import workloads
+    from rblas_amd.types import Compute
     n = args.size
     ld = n + 7
     flop = 2.0 * n * n * n
-    ms = time_gemm(np.float64, n, ld, args.steps, args.warmup)
+    log("dgemm ...")
+    ms = square_gemm(np.float64, n, ld, args.steps, args.warmup)
     tflops = flop / (ms * 1e-3) / 1e12
     out = {
-        "metric": "DGEMM 8192^3 TFLOP/s (f64, MFMA)",
+        "metric": METRIC,
         "value": round(tflops, 3),
         "unit": "TFLOP/s",
         "n_gpus": 1,
@@ -157,18 +385,46 @@ def main():
         "dtype": "f64",
         "data": "synthetic N(0,1), splitmix64 seed 1",
         "config": {"workload": f"dgemm col-major NN m=n=k={n} ld={ld} alpha=1.5 beta=-0.25",
-                   "pct_fp64_peak": round(100 * tflops / FP64_MFMA_PEAK_TFLOPS, 2)},
-        "roofline": {"bound": "mfma", "achieved": round(tflops, 3),
-                     "peak": FP64_MFMA_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": round(tflops / FP64_MFMA_PEAK_TFLOPS, 4),
+                   "pct_fp64_peak": round(100 * tflops / FP64_PEAK, 2)},
+        "roofline": {"bound": "mfma", "achieved": round(tflops, 3), "peak": FP64_PEAK,
+                     "unit": "TFLOP/s", "frac": round(tflops / FP64_PEAK, 4),
                      "traffic": measured_traffic()},
     }
-    if args.extra:
-        ms32 = time_gemm(np.float32, n, ld, args.steps, args.warmup)
-        t32 = flop / (ms32 * 1e-3) / 1e12
-        out["extra"] = {"sgemm_fp32_tflops": round(t32, 3),
-                        "sgemm_fp32_pct_peak": round(100 * t32 / FP32_MFMA_PEAK_TFLOPS, 2)}
+    if not args.quick:
+        comp = {"dgemm_8192_tflops": round(tflops, 3),
+                "dgemm_8192_pct_peak": round(100 * tflops / FP64_PEAK, 2)}
+        la = (n - 1) * ld + n
+        ops32 = (workloads.normal(1, 0, la, np.float32), workloads.normal(1, la, la, np.float32),
+                 workloads.normal(1, 2 * la, la, np.float32))
+        for name, mode in (("sgemm_fp32", None), ("sgemm_bf16x3", Compute.BF16X3),
+                           ("sgemm_bf16x6", Compute.BF16X6)):
+            log(f"{name} ...")
+            ms32 = square_gemm(np.float32, n, ld, 5, 2, compute=mode, operands=ops32)
+            t = flop / (ms32 * 1e-3) / 1e12
+            comp[f"{name}_8192_tflops"] = round(t, 2)
+            comp[f"{name}_8192_pct_fp32_peak"] = round(100 * t / FP32_PEAK, 2)
+        del ops32
+        comp["bf16x3_over_fp32_speed"] = round(comp["sgemm_bf16x3_8192_tflops"] /
+                                               comp["sgemm_fp32_8192_tflops"], 3)
+        comp["bf16x6_over_fp32_speed"] = round(comp["sgemm_bf16x6_8192_tflops"] /
+                                               comp["sgemm_fp32_8192_tflops"], 3)
+        log("precision ...")
+        comp["precision_err_over_bound"] = {c: precision_case(c) for c in ("a", "b", "c")}
+        comp["sgemm_skinny_1024x1024x65536_tflops"] = skinny_sgemm_tflops()
+        log("grouped gemm ...")
+        comp["grouped_gemm_f64"] = grouped_gemm(np.float64)
+        comp["grouped_gemm_f32"] = grouped_gemm(np.float32)
+        log("grouped trsm ...")
+        comp["grouped_trsm_f64"] = grouped_trsm()
+        log("serving ...")
+        comp["serve"] = serving(args.serve_requests)
+        comp["serve_p99_ms"] = comp["serve"]["batched"]["p99_ms"]
+        log("c64 / level-1 ...")
+        comp["c64_gemm_4096"] = c64_gemm()
+        comp["level1_f64"] = level1_bw()
+        out["components"] = comp
     if not args.no_cpu_baseline:
+        log("cpu baseline ...")
         out["cpu_baseline"] = cpu_baseline(n, ld)
     print(json.dumps(out), flush=True)
 

@@ -436,7 +436,7 @@ typedef struct {
 } rblas_serve_config_t;
 
 typedef struct {
-  double arrival_s, admit_s, done_s;
+  double arrival_s, admit_s, enqueued_s, done_s; /* enqueued: batch handed to the device */
   double flops;
   double footprint_bytes;
   uint64_t hash;

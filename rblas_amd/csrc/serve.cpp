@@ -48,6 +48,11 @@ struct Req {
   int esize = 8;                    // real element size of the operands
 };
 
+// A stream slot: one batch in flight at a time. Its device arena holds the
+// batch's operands, tables and workspaces; it is sized before the run starts
+// and reused batch after batch, so the timed loop allocates nothing (growing
+// a stream-ordered pool while other batches run stalls the host for as long
+// as the device is busy).
 struct Slot {
   hipStream_t st = nullptr;
   hipEvent_t ev = nullptr;
@@ -55,8 +60,33 @@ struct Slot {
   std::vector<int> reqs;
   char *pinned = nullptr;
   size_t pinned_cap = 0;
+  char *arena = nullptr;
+  size_t arena_cap = 0;
   size_t hash_off = 0;  // offset of the hash results in the staged block
 };
+
+// Make `cap` at least `need` (geometric growth); the slot must be idle.
+hipError_t ensure_arena(Slot &s, size_t need) {
+  if (need <= s.arena_cap) return hipSuccess;
+  if (s.arena) (void)hipFree(s.arena);
+  s.arena = nullptr;
+  const size_t cap = std::max(need + need / 4, 2 * s.arena_cap);
+  s.arena_cap = 0;
+  hipError_t e = hipMalloc((void **)&s.arena, cap);
+  if (e == hipSuccess) s.arena_cap = cap;
+  return e;
+}
+
+hipError_t ensure_pinned(Slot &s, size_t need) {
+  if (need <= s.pinned_cap) return hipSuccess;
+  if (s.pinned) (void)hipHostFree(s.pinned);
+  s.pinned = nullptr;
+  const size_t cap = std::max(need, std::max<size_t>(2 * s.pinned_cap, 1 << 20));
+  s.pinned_cap = 0;
+  hipError_t e = hipHostMalloc((void **)&s.pinned, cap, hipHostMallocDefault);
+  if (e == hipSuccess) s.pinned_cap = cap;
+  return e;
+}
 
 size_t round256(size_t x) { return (x + 255) & ~(size_t)255; }
 
@@ -125,9 +155,11 @@ struct Planner {
         }
       }
     }
+    const size_t ops_bytes = round256(total);
+    hipError_t e = hipSuccess;
+    // Operand addresses are arena offsets until the arena is sized (its size
+    // depends on the groups' workspaces); rebase() then adds its address.
     char *ops = nullptr;
-    hipError_t e = pool_alloc((void **)&ops, std::max<size_t>(total, 256), st);
-    if (e != hipSuccess) return e;
 
     // Host tables: generation, groups (batching mode), hash.
     std::vector<GenDesc> gens;
@@ -221,15 +253,16 @@ struct Planner {
     const size_t w_t32 = take(t32.units.empty() ? 0 : t32.ws_bytes());
     const size_t w_t64 = take(t64.units.empty() ? 0 : t64.ws_bytes());
     const size_t dev_bytes = off;
-    if (slot.pinned_cap < staged) {  // grows geometrically: host frees may synchronise
-      if (slot.pinned) (void)hipHostFree(slot.pinned);
-      slot.pinned = nullptr;
-      const size_t cap = std::max(staged, std::max<size_t>(2 * slot.pinned_cap, 1 << 20));
-      slot.pinned_cap = 0;
-      e = hipHostMalloc((void **)&slot.pinned, cap, hipHostMallocDefault);
-      if (e != hipSuccess) return e;
-      slot.pinned_cap = cap;
-    }
+    if ((e = ensure_pinned(slot, staged)) != hipSuccess) return e;
+    if ((e = ensure_arena(slot, ops_bytes + dev_bytes)) != hipSuccess) return e;
+    ops = slot.arena;
+    auto rebase = [&](auto *p) { return (decltype(p))((char *)p + (uintptr_t)ops); };
+    for (auto &x : g32.descs) x.a = rebase(x.a), x.b = rebase(x.b), x.c = rebase(x.c);
+    for (auto &x : g64.descs) x.a = rebase(x.a), x.b = rebase(x.b), x.c = rebase(x.c);
+    for (auto &x : t32.descs) x.a = rebase(x.a), x.b = rebase(x.b);
+    for (auto &x : t64.descs) x.a = rebase(x.a), x.b = rebase(x.b);
+    for (auto &x : gens) x.ptr = rebase((char *)x.ptr);
+    for (auto &x : hashes) x.ptr = rebase((const char *)x.ptr);
     char *h = slot.pinned;
     std::memcpy(h + o_gd, gens.data(), gens.size() * sizeof(GenDesc));
     std::memcpy(h + o_gu, gen_units.data(), gen_units.size() * sizeof(int2));
@@ -241,9 +274,7 @@ struct Planner {
     if (!t32.units.empty()) t32.stage(h + o_t32);
     if (!t64.units.empty()) t64.stage(h + o_t64);
     slot.hash_off = o_ho;
-    char *d = nullptr;
-    e = pool_alloc((void **)&d, std::max<size_t>(dev_bytes, 256), st);
-    if (e != hipSuccess) return e;
+    char *d = slot.arena + ops_bytes;
     e = hipMemcpyAsync(d, h, staged, hipMemcpyHostToDevice, st);
     if (e != hipSuccess) return e;
     e = launch_gen((const GenDesc *)(d + o_gd), (const int2 *)(d + o_gu), (int)gen_units.size(),
@@ -309,10 +340,7 @@ struct Planner {
         if (e != hipSuccess) return e;
       }
     }
-    e = hipFreeAsync(ops, st);
-    if (e == hipSuccess) e = hipFreeAsync(d, st);
-    if (e == hipSuccess) e = hipEventRecord(slot.ev, st);
-    return e;
+    return hipEventRecord(slot.ev, st);
   }
 
   template <typename T>
@@ -362,10 +390,25 @@ extern "C" int32_t rblas_serve_run(const rblas_serve_request_t *reqs_in, size_t 
       return fail(bad_arg("serve: request " + std::to_string(i) + " exceeds the HBM cap alone"),
                   nullptr);
   }
+  // The largest batch the FIFO policy can form: at most max_batch
+  // consecutive requests whose footprints fit the cap together.
+  double window = 0;
+  for (size_t i = 0; i < count; ++i) {
+    double sum = 0;
+    for (size_t j = i; j < count && j < i + (size_t)max_batch; ++j) {
+      if (sum + reqs[j].footprint > cfg->hbm_cap_bytes) break;
+      sum += reqs[j].footprint;
+    }
+    window = std::max(window, sum);
+  }
   std::vector<Slot> slots(slots_n);
   for (Slot &s : slots) {
     RBLAS_HIP_TRY(hipStreamCreateWithFlags(&s.st, hipStreamNonBlocking), "serve stream");
     RBLAS_HIP_TRY(hipEventCreateWithFlags(&s.ev, hipEventDisableTiming), "serve event");
+    // Operands plus room for tables and workspaces (the arena grows if a
+    // batch needs more).
+    RBLAS_HIP_TRY(ensure_arena(s, (size_t)(window * 1.05) + (64u << 20)), "serve arena");
+    RBLAS_HIP_TRY(ensure_pinned(s, 8u << 20), "serve staging");
   }
   Planner planner{*cfg, reqs, plain};
   std::deque<int> queue;
@@ -429,6 +472,8 @@ extern "C" int32_t rblas_serve_run(const rblas_serve_request_t *reqs_in, size_t 
         status = hip_fail(e, "serve launch");
         break;
       }
+      const double enq = seconds_since(t0);
+      for (int id : ids) results[id].enqueued_s = enq;
       s.reqs = ids;
       s.busy = true;
       ++batches;
@@ -441,6 +486,7 @@ extern "C" int32_t rblas_serve_run(const rblas_serve_request_t *reqs_in, size_t 
   }
   for (Slot &s : slots) {
     if (s.st) (void)hipStreamSynchronize(s.st);
+    if (s.arena) (void)hipFree(s.arena);
     if (s.pinned) (void)hipHostFree(s.pinned);
     if (s.ev) (void)hipEventDestroy(s.ev);
     if (s.st) (void)hipStreamDestroy(s.st);

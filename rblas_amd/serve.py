@@ -27,7 +27,8 @@ REQUEST_DTYPE = np.dtype([("kind", np.int32), ("dtype", np.int32), ("m", np.int3
                           ("n", np.int32), ("k", np.int32), ("pad", np.int32),
                           ("seed", np.uint64), ("arrival_s", np.float64)])
 RESULT_DTYPE = np.dtype([("arrival_s", np.float64), ("admit_s", np.float64),
-                         ("done_s", np.float64), ("flops", np.float64),
+                         ("enqueued_s", np.float64), ("done_s", np.float64),
+                         ("flops", np.float64),
                          ("footprint_bytes", np.float64), ("hash", np.uint64),
                          ("batch", np.int32), ("pad", np.int32)])
 
