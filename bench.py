@@ -56,6 +56,8 @@ def parse():
     p.add_argument("--quick", action="store_true", help="headline DGEMM only")
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--serve-requests", type=int, default=10000)
+    p.add_argument("--only", choices=sorted(ONLY),
+                   help="run one component and print its JSON (for profiling)")
     p.add_argument("--copy-calibrate", action="store_true",
                    help="only run 5 device-to-device copies of 1 GiB (counter calibration)")
     return p.parse_args()
@@ -354,8 +356,26 @@ def cpu_baseline(n, ld):
                       f"median of 3, oracle restatement of rblas's threaded CPU gemm"}
 
 
+ONLY = {
+    "sgemm_bf16x3": lambda a: {"ms": square_gemm(np.float32, a.size, a.size + 7, 5, 2,
+                                                 compute=1)},
+    "sgemm_bf16x6": lambda a: {"ms": square_gemm(np.float32, a.size, a.size + 7, 5, 2,
+                                                 compute=2)},
+    "sgemm_fp32": lambda a: {"ms": square_gemm(np.float32, a.size, a.size + 7, 5, 2)},
+    "grouped_gemm_f64": lambda a: grouped_gemm(np.float64),
+    "grouped_gemm_f32": lambda a: grouped_gemm(np.float32),
+    "grouped_trsm_f64": lambda a: grouped_trsm(),
+    "serve": lambda a: serving(a.serve_requests),
+    "c64_gemm": lambda a: c64_gemm(),
+    "level1": lambda a: level1_bw(),
+}
+
+
 def main():
     args = parse()
+    if args.only:
+        print(json.dumps({args.only: ONLY[args.only](args)}), flush=True)
+        return
     if args.copy_calibrate:
         copy_calibrate()
         return
