@@ -367,6 +367,7 @@ ONLY = {
     "grouped_trsm_f64": lambda a: grouped_trsm(),
     "serve": lambda a: serving(a.serve_requests),
     "c64_gemm": lambda a: c64_gemm(),
+    "sgemm_skinny": lambda a: skinny_sgemm_tflops(),
     "level1": lambda a: level1_bw(),
 }
 

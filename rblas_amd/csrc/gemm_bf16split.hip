@@ -22,6 +22,8 @@
 //                  by buffer_load_dwordx4 ... lds (3 LDS stages, one barrier per
 //                  16 k) and each fragment is reused in registers by every
 //                  product it takes part in (96 MFMAs per barrier for BF16x6).
+#include <algorithm>
+
 #include "kernels.hpp"
 
 namespace rblas {
@@ -137,6 +139,10 @@ struct SplitArgs {
   float alpha, beta;
   float *c;
   int64_t ldc;
+  int splits;        // k splits per tile (1 = none)
+  int split_ktiles;  // k-tiles per split (a multiple of kStages)
+  float *partials;   // splits > 1: kTile * kTile floats per (tile, split)
+  int *counters;     // splits > 1: one zero-initialised int per tile
 };
 
 template <int N>
@@ -148,11 +154,10 @@ __device__ __forceinline__ void stage_barrier() {
   asm volatile("" ::: "memory");
 }
 
-__device__ __forceinline__ void tile_coords(int tiles_m, int tiles_n, int &tm, int &tn) {
+__device__ __forceinline__ void tile_coords(int t, int tiles_m, int tiles_n, int &tm, int &tn) {
   constexpr int kXcds = 8;
   constexpr int kGroupM = 8;
   const int total = tiles_m * tiles_n;
-  int t = blockIdx.x;
   if (total % kXcds == 0) t = (t % kXcds) * (total / kXcds) + t / kXcds;
   const int per_group = kGroupM * tiles_n;
   const int group = t / per_group;
@@ -265,12 +270,33 @@ struct SplitGemm {
   }
 };
 
+// Workgroup u runs k split u / tiles of tile u % tiles. With splits, every
+// split stores its raw accumulators (register order: value e of thread t at
+// e * 256 + t of its slot), publishes them (device-scope release) and counts
+// itself on the tile's counter; the split that counts last (acquire) sums
+// the slots in split order -- a fixed order, so the bits do not depend on
+// which workgroup finishes last -- runs the epilogue and resets the counter.
 template <int PL>
 __global__ __launch_bounds__(kThreads) void split_gemm_kernel(SplitArgs p) {
+  __shared__ int s_last;
   SplitGemm<PL> g;
   const int tiles_m = (p.m + kTile - 1) / kTile, tiles_n = (p.n + kTile - 1) / kTile;
-  int tm, tn;
-  tile_coords(tiles_m, tiles_n, tm, tn);
+  const int tiles = tiles_m * tiles_n;
+  int u = blockIdx.x, tm, tn;
+  if (p.splits > 1) {
+    // Workgroups go round-robin over the 8 XCDs. Regroup ids so each XCD
+    // runs whole splits: the tiles of one split share its k-range of the A
+    // and B planes, which then stream once into that XCD's L2.
+    const int total = tiles * p.splits;
+    if (total % 8 == 0) u = (u % 8) * (total / 8) + u / 8;
+  }
+  const int split = u / tiles, tile = u % tiles;
+  if (p.splits > 1) {
+    tm = tile % tiles_m;
+    tn = tile / tiles_m;
+  } else {
+    tile_coords(tile, tiles_m, tiles_n, tm, tn);
+  }
   const int m0 = tm * kTile, n0 = tn * kTile;
   const int tid = threadIdx.x;
   g.wave = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -278,20 +304,60 @@ __global__ __launch_bounds__(kThreads) void split_gemm_kernel(SplitArgs p) {
   g.wm = (g.wave & 1) * kWave;
   g.wn = (g.wave >> 1) * kWave;
   g.ktiles = p.Kp / kBK;
+  const int kt0 = min(g.ktiles, split * p.split_ktiles);
+  const int kt1 = min(g.ktiles, kt0 + p.split_ktiles);
   g.init(p, m0, n0);
 #pragma unroll
   for (int i = 0; i < 4; ++i)
 #pragma unroll
     for (int j = 0; j < 4; ++j) g.acc[i][j] = f32x16{};
 
-  g.issue(0, lds_a<0>(), lds_b<0>());
-  g.issue(1, lds_a<1>(), lds_b<1>());
-  for (int kt = 0; kt < g.ktiles; kt += kStages) {
-    g.template step<0>(kt);
-    g.template step<1>(kt + 1);
-    g.template step<2>(kt + 2);
+  if (kt0 < kt1) {
+    // Every run is a multiple of kStages tiles; the two tiles prefetched past
+    // kt1 lie in the next split's range or in the planes' spare k-blocks.
+    g.issue(kt0, lds_a<0>(), lds_b<0>());
+    g.issue(kt0 + 1, lds_a<1>(), lds_b<1>());
+    for (int kt = kt0; kt < kt1; kt += kStages) {
+      g.template step<0>(kt);
+      g.template step<1>(kt + 1);
+      g.template step<2>(kt + 2);
+    }
+    stage_barrier<0>();  // the spare tiles' loads land before the LDS is reused or freed
   }
-  stage_barrier<0>();  // the spare tiles' loads land before the workgroup exits
+
+  if (p.splits > 1) {
+    constexpr int kSlot = kTile * kTile;
+    float *mine = p.partials + ((int64_t)tile * p.splits + split) * kSlot + threadIdx.x;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) mine[((i * 4 + j) * 16 + r) * kThreads] = g.acc[i][j][r];
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) s_last = atomicAdd(&p.counters[tile], 1) == p.splits - 1;
+    __syncthreads();
+    if (!s_last) return;
+    __threadfence();
+    // acc := slot 0 + slot 1 + ... in split order (this split's own slot is
+    // read back too, so the sum is the same whichever split finishes last).
+    // Splits outermost: each pass issues all of its loads at once.
+    const float *base = p.partials + (int64_t)tile * p.splits * kSlot + threadIdx.x;
+    for (int q = 0; q < p.splits; ++q) {
+      const float *slot = base + (int64_t)q * kSlot;
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+#pragma unroll
+          for (int r = 0; r < 16; ++r) {
+            const float v = slot[((i * 4 + j) * 16 + r) * kThreads];
+            g.acc[i][j][r] = q == 0 ? v : g.acc[i][j][r] + v;
+          }
+    }
+    if (threadIdx.x == 0) atomicExch(&p.counters[tile], 0);
+  }
 
   // Element r of lane l in block (i, j): C row m0+wm+32i+l%32, column
   // n0+wn+32j+8(r/4)+4(l/32)+r%4. Each block's C values are loaded together
@@ -341,11 +407,45 @@ void launch_split(bool r_contig, const float *x, int64_t ld, int rows, int K, in
                        rows_pad, Kp, planes, plane_elems);
 }
 
+__global__ void zero_counters(int *x, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) x[i] = 0;
+}
+
+// k splits of a split SGEMM: when the tile grid fills under half the CUs and
+// each split keeps at least 64 k-tiles, enough splits to give ~256
+// workgroups (at most 16). A function of the shape only.
+struct SplitK {
+  int splits, per;
+};
+SplitK split_k(int64_t m, int64_t n, int64_t k) {
+  const int64_t tiles = (round_up(m, kTile) / kTile) * (round_up(n, kTile) / kTile);
+  const int64_t kt = padded_k(k) / kBK;
+  int64_t s = 1;
+  if (tiles < kCUs / 2) {
+    s = std::min<int64_t>((kCUs + tiles - 1) / tiles, kt / 64);
+    s = std::max<int64_t>(1, std::min<int64_t>(s, 16));
+  }
+  const int64_t per = round_up((kt + s - 1) / s, kStages);
+  s = (kt + per - 1) / per;
+  return SplitK{(int)s, (int)per};
+}
+
+size_t planes_bytes(int64_t m, int64_t n, int64_t k, int planes) {
+  const int64_t ka = alloc_k(k), mp = round_up(m, kTile), np = round_up(n, kTile);
+  return (size_t)planes * (size_t)(mp + np) * (size_t)ka * 2;
+}
+
 }  // namespace
 
 size_t sgemm_split_workspace(int64_t m, int64_t n, int64_t k, int planes) {
-  const int64_t ka = alloc_k(k), mp = round_up(m, kTile), np = round_up(n, kTile);
-  return (size_t)planes * (size_t)(mp + np) * (size_t)ka * 2;
+  const SplitK sk = split_k(m, n, k);
+  size_t bytes = round_up((int64_t)planes_bytes(m, n, k, planes), 256);
+  if (sk.splits > 1) {
+    const int64_t tiles = (round_up(m, kTile) / kTile) * (round_up(n, kTile) / kTile);
+    bytes += (size_t)tiles * sk.splits * kTile * kTile * sizeof(float) + tiles * sizeof(int);
+  }
+  return bytes;
 }
 
 hipError_t launch_sgemm_split(const GemmProblem<float> &p, int planes, void *workspace,
@@ -367,13 +467,23 @@ hipError_t launch_sgemm_split(const GemmProblem<float> &p, int planes, void *wor
   }
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  SplitArgs a{wa, wb, pa, pb, (int)p.m, (int)p.n, (int)mp, (int)np, (int)kp, p.alpha, p.beta,
-              p.c, p.ldc};
+  const SplitK sk = split_k(p.m, p.n, p.k);
   const int64_t tiles = (mp / kTile) * (np / kTile);
+  float *partials = nullptr;
+  int *counters = nullptr;
+  if (sk.splits > 1) {
+    partials = (float *)((char *)workspace + round_up((int64_t)planes_bytes(p.m, p.n, p.k, planes), 256));
+    counters = (int *)(partials + tiles * sk.splits * kTile * kTile);
+    hipLaunchKernelGGL(zero_counters, dim3((unsigned)((tiles + 255) / 256)), dim3(256), 0, stream,
+                       counters, (int)tiles);
+  }
+  SplitArgs a{wa, wb, pa, pb, (int)p.m, (int)p.n, (int)mp, (int)np, (int)kp, p.alpha, p.beta,
+              p.c, p.ldc, sk.splits, sk.per, partials, counters};
+  const dim3 grid((unsigned)(tiles * sk.splits));
   if (planes == 2)
-    hipLaunchKernelGGL((split_gemm_kernel<2>), dim3((unsigned)tiles), dim3(kThreads), 0, stream, a);
+    hipLaunchKernelGGL((split_gemm_kernel<2>), grid, dim3(kThreads), 0, stream, a);
   else
-    hipLaunchKernelGGL((split_gemm_kernel<3>), dim3((unsigned)tiles), dim3(kThreads), 0, stream, a);
+    hipLaunchKernelGGL((split_gemm_kernel<3>), grid, dim3(kThreads), 0, stream, a);
   return hipGetLastError();
 }
 

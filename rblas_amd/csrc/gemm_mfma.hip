@@ -14,6 +14,8 @@
 //     in registers;
 //   - materialize (level3.rs:179-222) -> none: each operand is staged from
 //     its stored orientation (M/N-contiguous or K-contiguous).
+#include <algorithm>
+
 #include "tile_gemm.hpp"
 
 namespace rblas {
@@ -121,22 +123,22 @@ GemmConfig gemm_config(const GemmProblem<T> &p) {
   const int64_t t128 = ((p.m + 127) / 128) * ((p.n + 127) / 128);
   const int64_t t64 = ((p.m + 63) / 64) * ((p.n + 63) / 64);
   const int64_t ktiles = (p.k + Geo<T>::BK - 1) / Geo<T>::BK;
-  int64_t tiles = t128;
-  if (t128 < kCUs) {
-    cfg.tile = 64;
-    tiles = t64;
+  if (t128 >= kCUs) return cfg;
+  // Under one 128-tile per CU: long k is better served by 128-tiles with k
+  // split until the grid fills the CUs (each split keeping >= 64 k-tiles, so
+  // the partial traffic stays small); otherwise 64-tiles, split as well when
+  // even they leave most CUs idle.
+  int64_t s = std::min<int64_t>((kCUs + t128 - 1) / t128, ktiles / 64);
+  s = std::min<int64_t>(s, 16);
+  if (s >= 2 && t128 * s >= kCUs / 2) {
+    cfg.splits = (int)s;
+    return cfg;
   }
-  // Split k when the tile grid leaves most CUs idle and each split keeps at
-  // least 64 k-tiles of work (so the partial traffic stays small).
-  if (tiles < kCUs / 2) {
-    int64_t s = (kCUs + tiles - 1) / tiles;
-    if (s > ktiles / 64) s = ktiles / 64;
-    if (s > 16) s = 16;
-    if (s >= 2) {
-      cfg.splits = (int)s;
-      // With splits the bigger tile is the better tile again.
-      if (cfg.tile == 64 && t128 * s >= kCUs / 2) cfg.tile = 128;
-    }
+  cfg.tile = 64;
+  if (t64 < kCUs / 2) {
+    s = std::min<int64_t>((kCUs + t64 - 1) / t64, ktiles / 64);
+    s = std::min<int64_t>(s, 16);
+    if (s >= 2) cfg.splits = (int)s;
   }
   return cfg;
 }
@@ -173,6 +175,9 @@ hipError_t launch_gemm_cfg(const GemmProblem<T> &p, GemmConfig cfg, void *ws, hi
 template <typename T>
 hipError_t launch_gemm(const GemmProblem<T> &p, hipStream_t st) {
   GemmConfig cfg = gemm_config(p);
+  // Without a workspace there is no k split: where the config wanted one
+  // (fewer than one 128-tile per CU), 64-tiles give 4x the workgroups.
+  if (cfg.splits > 1) cfg.tile = 64;
   cfg.splits = 1;
   return launch_gemm_cfg(p, cfg, nullptr, st);
 }

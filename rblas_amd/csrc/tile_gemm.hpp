@@ -481,27 +481,24 @@ struct TileGemm {
         for (int r = 0; r < 4; ++r) base[((i * FN + j) * 4 + r) * kThreads] = acc[i][j][r];
   }
 
-  // acc := sum over s = 0 .. splits-1 of slot (slot0 + s), added in that
-  // order (split `mine` uses its own registers, which hold the same values it
-  // stored): a fixed order, so the result does not depend on which
-  // workgroup finishes last.
-  __device__ __forceinline__ void reduce_partials(const T *ws, int64_t slot0, int splits,
-                                                  int mine) {
-    const T *base = ws + tid;
+  // acc := slot slot0 + slot slot0+1 + ... + slot slot0+splits-1, added in
+  // that order (this split's own slot is read back too): a fixed order, so
+  // the result does not depend on which workgroup finishes last. Splits are
+  // the outer loop, so each pass issues all of its loads at once.
+  __device__ __forceinline__ void reduce_partials(const T *ws, int64_t slot0, int splits) {
     const int64_t stride = (int64_t)(kAccPerThread * kThreads);
+    for (int s = 0; s < splits; ++s) {
+      const T *slot = ws + (slot0 + s) * stride + tid;
 #pragma unroll
-    for (int i = 0; i < FM; ++i)
+      for (int i = 0; i < FM; ++i)
 #pragma unroll
-      for (int j = 0; j < FN; ++j)
+        for (int j = 0; j < FN; ++j)
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const int e = ((i * FN + j) * 4 + r) * kThreads;
-          const T own = acc[i][j][r];
-          T sum = mine == 0 ? own : base[slot0 * stride + e];
-          for (int s = 1; s < splits; ++s)
-            sum += (s == mine) ? own : base[(slot0 + s) * stride + e];
-          acc[i][j][r] = sum;
-        }
+          for (int r = 0; r < 4; ++r) {
+            const T v = slot[((i * FN + j) * 4 + r) * kThreads];
+            acc[i][j][r] = s == 0 ? v : acc[i][j][r] + v;
+          }
+    }
   }
 };
 
@@ -524,7 +521,7 @@ __device__ __forceinline__ bool split_handoff(TG &tg, T *ws, int64_t slot0, int 
   __syncthreads();
   if (!s_last) return false;
   __threadfence();
-  tg.reduce_partials(ws, slot0, splits, mine);
+  tg.reduce_partials(ws, slot0, splits);
   if (threadIdx.x == 0) atomicExch(counter, 0);
   return true;
 }

@@ -102,3 +102,33 @@ def test_split_device_entry(gpu, compute):
     s.synchronize()
     assert np.array_equal(dc.download(s), ch)
     s.close()
+
+
+@pytest.mark.parametrize("compute", [Compute.BF16X3, Compute.BF16X6])
+@pytest.mark.parametrize("m,n,k,ta,tb", [(300, 260, 5000, 0, 0), (64, 40, 20000, 1, 1),
+                                         (1024, 256, 9000, 0, 1)])
+def test_split_k_exact_deterministic_and_bounded(gpu, compute, m, n, k, ta, tb):
+    """Shapes whose 256 x 256 tile grid fills under half the CUs split k over
+    several workgroups (ordered reduction of the partial tiles): exact on
+    small integers, the same bits on a second call, within the mode's bound."""
+    rng = np.random.default_rng(m + k)
+    a, lda, b, ldb, c, ldc = problem(COL, ta, tb, m, n, k, 3, 8)
+    ai, bi, ci = (rng.integers(-3, 4, x.size).astype(np.float32) for x in (a, b, c))
+    cc = ci.copy()
+    level3.sgemm_ex(COL, ta, tb, m, n, k, 1.0, ai, lda, bi, ldb, 0.5, cc, ldc, compute)
+    A = op(ai, COL, ta, m, k, lda).astype(np.float64)
+    B = op(bi, COL, tb, k, n, ldb).astype(np.float64)
+    want = A @ B + 0.5 * view(ci, COL, m, n, ldc)
+    assert np.array_equal(view(cc, COL, m, n, ldc).astype(np.float64), want)
+    c1, c2 = c.copy(), c.copy()
+    level3.sgemm_ex(COL, ta, tb, m, n, k, 1.5, a, lda, b, ldb, -0.25, c1, ldc, compute)
+    level3.sgemm_ex(COL, ta, tb, m, n, k, 1.5, a, lda, b, ldb, -0.25, c2, ldc, compute)
+    assert np.array_equal(c1, c2)
+    A = op(a, COL, ta, m, k, lda).astype(np.float64)
+    B = op(b, COL, tb, k, n, ldb).astype(np.float64)
+    C0 = view(c, COL, m, n, ldc).astype(np.float64)
+    want = 1.5 * (A @ B) - 0.25 * C0
+    mag = 1.5 * (np.abs(A) @ np.abs(B))
+    rep = 3 * 2.0 ** -16 if compute == Compute.BF16X3 else 0.0
+    tol = (rep + 2 * (k + 2) * U) * mag + 4 * U * 0.25 * np.abs(C0) + 1e-30
+    assert (np.abs(view(c1, COL, m, n, ldc) - want) <= tol).all()
