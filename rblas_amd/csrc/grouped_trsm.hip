@@ -120,30 +120,37 @@ __device__ void solve_unit(const TrsmDesc<T> &d, int c0, const double *inv, doub
     f64x4 acc = {0, 0, 0, 0};
     const int iw = wave;
     if (16 * iw < nb) {
-      // Off-diagonal 16 x 16 sub-blocks: j-blocks before iw (lower) or after.
-      const int jb_lo = d.upper ? iw + 1 : 0, jb_hi = d.upper ? 4 : iw;
-      for (int jb = jb_lo; jb < jb_hi; ++jb) {
-        if (16 * jb >= nb) break;
+      // Off-diagonal 16 x 16 sub-blocks: j-blocks before iw (lower) or after
+      // (upper); unrolled, the loads of a sub-block are issued together.
 #pragma unroll
-        for (int p = 0; p < 16; p += 4) {
-          const int j = 16 * jb + p + kq;
-          const double pv = ib[(int64_t)j * kNB + 16 * iw + (lane & 15)];
-          const double qv = (col < W) ? Bs[(r0 + j) * W + col] : 0.0;
-          acc = mma64(pv, qv, acc);
+      for (int jb = 0; jb < 4; ++jb) {
+        const bool use = (d.upper ? jb > iw : jb < iw) && 16 * jb < nb;  // wave-uniform
+        if (use) {
+          double pv[4], qv[4];
+#pragma unroll
+          for (int t4 = 0; t4 < 4; ++t4) {
+            const int j = 16 * jb + 4 * t4 + kq;
+            pv[t4] = ib[(int64_t)j * kNB + 16 * iw + (lane & 15)];
+            qv[t4] = (col < W) ? Bs[(r0 + j) * W + col] : 0.0;
+          }
+#pragma unroll
+          for (int t4 = 0; t4 < 4; ++t4) acc = mma64(pv[t4], qv[t4], acc);
         }
       }
-      // Diagonal sub-block: only the triangle, scalar f64.
+      // Diagonal sub-block: only the triangle, scalar f64 (predicated, so
+      // every lane runs the same 16 steps).
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int il = kq + 4 * r;  // row within the sub-block (MFMA output layout)
         const int i = 16 * iw + il;
         double sum = acc[r];
-        if (col < W) {
-          const int jl_lo = d.upper ? il : 0, jl_hi = d.upper ? 16 : il + 1;
-          for (int jl = jl_lo; jl < jl_hi; ++jl) {
-            const int j = 16 * iw + jl;
-            if (j < nb) sum = __builtin_fma(ib[(int64_t)j * kNB + i], Bs[(r0 + j) * W + col], sum);
-          }
+#pragma unroll
+        for (int jl = 0; jl < 16; ++jl) {
+          const int j = 16 * iw + jl;
+          const bool on = (d.upper ? jl >= il : jl <= il) && j < nb && col < W;
+          const double w = on ? ib[(int64_t)j * kNB + i] : 0.0;
+          const double x = on ? Bs[(r0 + j) * W + col] : 0.0;
+          sum = on ? __builtin_fma(w, x, sum) : sum;
         }
         acc[r] = sum;
       }
@@ -158,18 +165,31 @@ __device__ void solve_unit(const TrsmDesc<T> &d, int c0, const double *inv, doub
     }
     __syncthreads();  // X_b visible
     // B_rest -= op(A)[rest, b] * X_b, rest = rows after the block (lower) or
-    // before it (upper), in 16-row blocks spread over the waves.
+    // before it (upper), in 16-row blocks spread over the waves. Lane l reads
+    // op(A)(rb + l%16, r0 + 4t + l/16) for t = 0..15: all 16 loads of a
+    // row-block are issued before its 16 MFMAs.
     const int lo = d.upper ? 0 : r0 + nb, hi = d.upper ? r0 : q;
+    const int64_t si = d.trans ? d.lda : 1;  // op(A) row step
+    const int64_t sj = d.trans ? 1 : d.lda;  // op(A) column step
+    double xq[16];
+#pragma unroll
+    for (int t4 = 0; t4 < 16; ++t4) {
+      const int j = 4 * t4 + kq;
+      xq[t4] = (col < W && j < nb) ? Bs[(r0 + j) * W + col] : 0.0;
+    }
     for (int rb = lo + 16 * wave; rb < hi; rb += 64) {
-      f64x4 u = {0, 0, 0, 0};
       const int64_t ar = rb + (lane & 15);
       const bool row_ok = ar < hi;
-      for (int p = 0; p < nb; p += 4) {
-        const int j = p + kq;
-        const double pv = (row_ok && j < nb) ? opa(d, ar, r0 + j) : 0.0;
-        const double qv = (col < W && j < nb) ? Bs[(r0 + j) * W + col] : 0.0;
-        u = mma64(pv, qv, u);
+      const T *ap = d.a + ar * si + (int64_t)(r0 + kq) * sj;
+      double pv[16];
+#pragma unroll
+      for (int t4 = 0; t4 < 16; ++t4) {
+        const int j = 4 * t4 + kq;
+        pv[t4] = (row_ok && j < nb) ? (double)ap[(int64_t)(4 * t4) * sj] : 0.0;
       }
+      f64x4 u = {0, 0, 0, 0};
+#pragma unroll
+      for (int t4 = 0; t4 < 16; ++t4) u = mma64(pv[t4], xq[t4], u);
       if (col < W) {
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
