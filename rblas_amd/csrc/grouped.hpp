@@ -65,6 +65,17 @@ constexpr int kTrsmLdsDoubles = 16384;
 template <typename T>
 int grouped_trsm_grid();
 
+// One problem (d.inv0 = 0) without tables: one launch inverting its
+// diagonal blocks into `inv`, one launch of one workgroup per `width`
+// right-hand sides. Requires d.q * d.width <= kTrsmLdsDoubles.
+template <typename T>
+hipError_t launch_trsm_single(const TrsmDesc<T> &d, double *inv, hipStream_t stream);
+
+// Inverts every 64 x 64 diagonal block of op(A) (d.a, d.lda, d.q, d.trans,
+// d.upper, d.unit; other fields unused) into inv + d.inv0, 4096 f64 each.
+template <typename T>
+hipError_t launch_trsm_invert(const TrsmDesc<T> &d, double *inv, hipStream_t stream);
+
 // Inverts the diagonal blocks listed in `blocks` ({problem, block}), then
 // runs one persistent launch over `units` ({problem, first right-hand
 // side}). sched[0..1] as for launch_grouped_gemm; `inv` holds 4096 f64 per

@@ -22,9 +22,17 @@
 namespace rblas {
 namespace {
 
-constexpr int kThreads = 256;       // 4 waves
+#ifndef RBLAS_TRSM_WAVES
+#define RBLAS_TRSM_WAVES 8
+#endif
+constexpr int kWaves = RBLAS_TRSM_WAVES;  // solve workgroup: 8 waves
+constexpr int kThreads = 64 * kWaves;
+constexpr int kInvThreads = 1024;   // inversion workgroup: 16 waves, 4 columns each
 constexpr int kNB = 64;             // diagonal block
 constexpr int kLdsDoubles = kTrsmLdsDoubles;  // q * W limit (128 KiB)
+constexpr int kSub = 256;           // doubles in a 16 x 16 sub-block
+constexpr int kInvLds = 10 * kSub;  // the 10 sub-blocks of a triangular 64 x 64 block
+constexpr int kInvPerThread = (kInvLds + kThreads - 1) / kThreads;
 
 typedef double f64x4 __attribute__((ext_vector_type(4)));
 
@@ -39,48 +47,80 @@ __device__ __forceinline__ double opa(const TrsmDesc<T> &d, int64_t r, int64_t c
 
 __device__ __forceinline__ int uni(int x) { return __builtin_amdgcn_readfirstlane(x); }
 
+// Value of v in lane l (l wave-uniform), in every lane.
+__device__ __forceinline__ double lane_value(double v, int l) {
+  const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
+  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)b, l);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), l);
+  return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+
 // Inverse of diagonal block `blk` of op(A) (as the problem is solved: Left
-// form), f64, column-major 64 x 64 at inv + d.inv0 + blk * 4096. Thread j
-// builds column j by substitution against e_j.
+// form), f64, column-major 64 x 64 at inv + d.inv0 + blk * 4096, zero outside
+// the block's nb x nb corner. An upper block is inverted as the lower one
+// P U P (P reverses the nb indices). Column-oriented substitution against
+// the identity, in registers: lane i holds row i, wave w columns 4w..4w+3;
+// step k divides row k by L(k, k) and subtracts L(i, k) times it from every
+// later row i. Column j only takes part from step j on (earlier steps would
+// only subtract multiples of its zeros), so each entry gets the same
+// operations, in the same order, as a row-by-row substitution of that column.
 template <typename T>
-__global__ __launch_bounds__(kNB) void grouped_trsm_inv(const TrsmDesc<T> *descs,
-                                                        const int2 *blocks, double *inv) {
-  __shared__ double L[kNB][kNB + 1];
-  __shared__ double X[kNB][kNB + 1];
-  const int2 u = blocks[blockIdx.x];
-  const TrsmDesc<T> d = descs[u.x];
-  const int64_t r0 = (int64_t)u.y * kNB;
+__device__ __forceinline__ void invert_block(const TrsmDesc<T> &d, int blk, double *inv) {
+  __shared__ double Ls[kNB][kNB + 1];  // Ls[k][i] = L(i, k), lower form
+  const int64_t r0 = (int64_t)blk * kNB;
   const int nb = (int)min((int64_t)kNB, (int64_t)d.q - r0);
   const int t = threadIdx.x;
-  for (int idx = t; idx < kNB * kNB; idx += kNB) {
-    const int r = idx % kNB, c = idx / kNB;
-    double v = 0.0;
-    const bool in_tri = d.upper ? r <= c : r >= c;
-    if (r < nb && c < nb && in_tri) v = (d.unit && r == c) ? 1.0 : opa(d, r0 + r, r0 + c);
-    L[r][c] = v;
-    X[r][c] = 0.0;
+  for (int idx = t; idx < kNB * kNB; idx += kInvThreads) {
+    const int i = idx % kNB, k = idx / kNB;
+    double v = i == k ? 1.0 : 0.0;  // padding beyond nb: identity
+    if (i < nb && k < nb && i >= k) {
+      const int r = d.upper ? nb - 1 - i : i, c = d.upper ? nb - 1 - k : k;
+      v = (d.unit && i == k) ? 1.0 : opa(d, r0 + r, r0 + c);
+    }
+    Ls[k][i] = v;
   }
   __syncthreads();
-  const int j = t;
-  if (j < nb) {
-    X[j][j] = 1.0 / L[j][j];
-    if (!d.upper) {
-      for (int i = j + 1; i < nb; ++i) {
-        double s = 0.0;
-        for (int kk = j; kk < i; ++kk) s = __builtin_fma(L[i][kk], X[kk][j], s);
-        X[i][j] = -s / L[i][i];
-      }
-    } else {
-      for (int i = j - 1; i >= 0; --i) {
-        double s = 0.0;
-        for (int kk = i + 1; kk <= j; ++kk) s = __builtin_fma(L[i][kk], X[kk][j], s);
-        X[i][j] = -s / L[i][i];
+  const int lane = t & 63, w = uni(t >> 6);
+  const int j0 = 4 * w;
+  double x[4];
+#pragma unroll
+  for (int c = 0; c < 4; ++c) x[c] = lane == j0 + c ? 1.0 : 0.0;
+  for (int k = j0; k < kNB; ++k) {
+    const double lkk = Ls[k][k];
+    const double lik = Ls[k][lane];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      if (k >= j0 + c) {  // wave-uniform
+        const double xk = lane_value(x[c], k) / lkk;
+        if (lane == k)
+          x[c] = xk;
+        else if (lane > k)
+          x[c] = __builtin_fma(-lik, xk, x[c]);
       }
     }
   }
-  __syncthreads();
-  double *out = inv + d.inv0 + (int64_t)u.y * kNB * kNB;
-  for (int idx = t; idx < kNB * kNB; idx += kNB) out[idx] = X[idx % kNB][idx / kNB];
+  double *out = inv + d.inv0 + (int64_t)blk * kNB * kNB;
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    const int jl = j0 + c;
+    const bool in = lane < nb && jl < nb;
+    const int i = (d.upper && lane < nb) ? nb - 1 - lane : lane;
+    const int j = (d.upper && jl < nb) ? nb - 1 - jl : jl;
+    out[i + j * kNB] = in ? x[c] : 0.0;
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kInvThreads) void grouped_trsm_inv(const TrsmDesc<T> *descs,
+                                                                const int2 *blocks, double *inv) {
+  const int2 u = blocks[blockIdx.x];
+  invert_block(descs[u.x], u.y, inv);
+}
+
+// One problem passed by value: block blockIdx.x.
+template <typename T>
+__global__ __launch_bounds__(kInvThreads) void single_trsm_inv(TrsmDesc<T> d, double *inv) {
+  invert_block(d, blockIdx.x, inv);
 }
 
 // Address of right-hand side c, row r of the unit's B (Left: B(r, c0 + c);
@@ -90,13 +130,92 @@ __device__ __forceinline__ int64_t b_index(const TrsmDesc<T> &d, int64_t r, int6
   return d.right ? c + r * d.ldb : r + c * d.ldb;
 }
 
-// Bs: the kernel's LDS block of kLdsDoubles doubles, used as Bs[r * W + c].
+// LDS copy of a diagonal-block inverse: its 10 nonzero 16 x 16 sub-blocks,
+// each column-major. Sub-block (bi, bj) of a lower block (bi >= bj) sits at
+// slot bi (bi + 1) / 2 + bj; an upper block uses the slot of (3 - bi, 3 - bj).
+__device__ __forceinline__ int sub_slot(bool upper, int bi, int bj) {
+  if (upper) {
+    bi = 3 - bi;
+    bj = 3 - bj;
+  }
+  return (bi * (bi + 1) / 2 + bj) * kSub;
+}
+
+// Element e (< kInvLds) of the LDS copy: its offset in the 64 x 64 inverse.
+__device__ __forceinline__ int inv_src(bool upper, int e) {
+  const int slot = e / kSub, w = e % kSub;
+  // slot -> (bi, bj) of the lower form: 0:(0,0) 1:(1,0) 2:(1,1) 3:(2,0) ...
+  const int bi = slot >= 6 ? 3 : slot >= 3 ? 2 : slot >= 1 ? 1 : 0;
+  int bj = slot - bi * (bi + 1) / 2;
+  int ri = bi;
+  if (upper) {
+    ri = 3 - bi;
+    bj = 3 - bj;
+  }
+  return (16 * ri + w % 16) + (16 * bj + w / 16) * kNB;
+}
+
+// Raw loads of op(A)(rb + l%16, r0 + 4t + l/16), t = 0..15, for lane l. An
+// entry outside rows [.., hi) or block columns [0, nb) loads op(A)(0, 0)
+// instead (a valid address), so every load is issued without a branch; the
+// out-of-range entries are zeroed where they are used (update_rows), which
+// leaves the loads in flight until then.
+template <typename T>
+__device__ __forceinline__ void load_rows(const TrsmDesc<T> &d, int rb, int hi, int r0, int nb,
+                                          int64_t si, int64_t sj, int lane, T (&v)[16]) {
+  const int kq = lane >> 4;
+  const int64_t ar = rb + (lane & 15);
+  const bool row_ok = ar < hi;
+  const T *ap = d.a + ar * si + (int64_t)(r0 + kq) * sj;
+#pragma unroll
+  for (int t4 = 0; t4 < 16; ++t4) {
+    const bool ok = row_ok && 4 * t4 + kq < nb;
+    v[t4] = *(ok ? ap + (int64_t)(4 * t4) * sj : d.a);
+  }
+}
+
+// Bs[i][col] -= (op(A)[rb .. rb+15, block] * X_b)[i][col] for rows i < hi,
+// from the raw loads of load_rows.
+template <typename T>
+__device__ __forceinline__ void update_rows(double *Bs, const T (&pv)[16], const double (&xq)[16],
+                                            int rb, int hi, int nb, int W, int lane) {
+  const int col = lane & 15, kq = lane >> 4;
+  const bool row_ok = rb + (lane & 15) < hi;
+  f64x4 u = {0, 0, 0, 0};
+#pragma unroll
+  for (int t4 = 0; t4 < 16; ++t4) {
+    const bool ok = row_ok && 4 * t4 + kq < nb;
+    u = mma64(ok ? (double)pv[t4] : 0.0, xq[t4], u);
+  }
+  if (col < W) {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int i = rb + kq + 4 * r;
+      if (i < hi) Bs[i * W + col] -= u[r];
+    }
+  }
+}
+
+// Bs: the kernel's LDS block of kLdsDoubles doubles, used as Bs[r * W + c];
+// Is: kInvLds doubles for the current diagonal-block inverse.
+//
+// Per diagonal block b, in substitution order:
+//   1. every wave issues the op(A) loads of its first 16-row block of the
+//      update (they do not depend on X_b);
+//   2. waves 0-3 form X_b = inv_b * B_b from LDS (rows 16w .. 16w + 15);
+//   3. X_b replaces B_b in LDS;
+//   4. every wave issues the loads of inv_{b+1}, then runs the update
+//      B_rest -= op(A)[rest, b] X_b over its 16-row blocks (stride 16 x
+//      kWaves rows), loading the next row block before the MFMAs of this one;
+//   5. inv_{b+1} goes to LDS.
 template <typename T, int W>
-__device__ void solve_unit(const TrsmDesc<T> &d, int c0, const double *inv, double *Bs) {
-  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+__device__ void solve_unit(const TrsmDesc<T> &d, int c0, const double *inv, double *Bs,
+                           double *Is) {
+  const int t = threadIdx.x, lane = t & 63, wave = uni(t >> 6);
   const int q = d.q;
   const int nc = min(W, d.nrhs - c0);
   const int qpad = (q + kNB - 1) / kNB * kNB;
+  const bool upper = d.upper;
   // Load (alpha applied to every entry, as rblas's pre-scale: 0 * NaN = NaN).
   for (int e = t; e < qpad * W; e += kThreads) {
     const int r = e / W, c = e % W;
@@ -107,95 +226,108 @@ __device__ void solve_unit(const TrsmDesc<T> &d, int c0, const double *inv, doub
     }
     Bs[e] = v;
   }
-  __syncthreads();
   const int nblk = qpad / kNB;
-  const int col = lane & 15;          // MFMA output column / Q-operand column
-  const int kq = lane >> 4;           // k within a 4-step
+  {
+    const double *ib = inv + d.inv0 + (int64_t)(d.forward ? 0 : nblk - 1) * kNB * kNB;
+    for (int e = t; e < kInvLds; e += kThreads) Is[e] = ib[inv_src(upper, e)];
+  }
+  __syncthreads();
+  const int col = lane & 15;  // MFMA output column / Q-operand column
+  const int kq = lane >> 4;   // k within a 4-step
+  const int64_t si = d.trans ? d.lda : 1;  // op(A) row step
+  const int64_t sj = d.trans ? 1 : d.lda;  // op(A) column step
   for (int s = 0; s < nblk; ++s) {
     const int b = d.forward ? s : nblk - 1 - s;
     const int r0 = b * kNB;
     const int nb = min(kNB, q - r0);
-    const double *ib = inv + d.inv0 + (int64_t)b * kNB * kNB;
-    // X_b = inv_b * B_b: wave w owns rows 16w .. 16w+15 of the block.
+    const int lo = upper ? 0 : r0 + nb, hi = upper ? r0 : q;
+    // 1. First update row block of this wave.
+    int rb = lo + 16 * wave;
+    T pa[16], pb[16];
+    if (rb < hi) load_rows(d, rb, hi, r0, nb, si, sj, lane, pa);
+    // 2. X_b = inv_b * B_b: wave w < 4 owns rows 16w .. 16w+15 of the block.
     f64x4 acc = {0, 0, 0, 0};
     const int iw = wave;
-    if (16 * iw < nb) {
-      // Off-diagonal 16 x 16 sub-blocks: j-blocks before iw (lower) or after
-      // (upper); unrolled, the loads of a sub-block are issued together.
+    const bool xw = iw < 4 && 16 * iw < nb;
+    if (xw) {
 #pragma unroll
       for (int jb = 0; jb < 4; ++jb) {
-        const bool use = (d.upper ? jb > iw : jb < iw) && 16 * jb < nb;  // wave-uniform
+        const bool use = (upper ? jb > iw : jb < iw) && 16 * jb < nb;  // wave-uniform
         if (use) {
-          double pv[4], qv[4];
+          const double *sb = Is + sub_slot(upper, iw, jb);
+          double pa[4], qa[4];
 #pragma unroll
           for (int t4 = 0; t4 < 4; ++t4) {
-            const int j = 16 * jb + 4 * t4 + kq;
-            pv[t4] = ib[(int64_t)j * kNB + 16 * iw + (lane & 15)];
-            qv[t4] = (col < W) ? Bs[(r0 + j) * W + col] : 0.0;
+            const int j = 4 * t4 + kq;
+            pa[t4] = sb[(lane & 15) + 16 * j];
+            qa[t4] = (col < W) ? Bs[(r0 + 16 * jb + j) * W + col] : 0.0;
           }
 #pragma unroll
-          for (int t4 = 0; t4 < 4; ++t4) acc = mma64(pv[t4], qv[t4], acc);
+          for (int t4 = 0; t4 < 4; ++t4) acc = mma64(pa[t4], qa[t4], acc);
         }
       }
       // Diagonal sub-block: only the triangle, scalar f64 (predicated, so
       // every lane runs the same 16 steps).
+      const double *sd = Is + sub_slot(upper, iw, iw);
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int il = kq + 4 * r;  // row within the sub-block (MFMA output layout)
-        const int i = 16 * iw + il;
         double sum = acc[r];
 #pragma unroll
         for (int jl = 0; jl < 16; ++jl) {
           const int j = 16 * iw + jl;
-          const bool on = (d.upper ? jl >= il : jl <= il) && j < nb && col < W;
-          const double w = on ? ib[(int64_t)j * kNB + i] : 0.0;
+          const bool on = (upper ? jl >= il : jl <= il) && j < nb && col < W;
+          const double w = on ? sd[il + 16 * jl] : 0.0;
           const double x = on ? Bs[(r0 + j) * W + col] : 0.0;
           sum = on ? __builtin_fma(w, x, sum) : sum;
         }
         acc[r] = sum;
       }
     }
-    __syncthreads();  // every wave has read B_b
-    if (16 * iw < nb && col < W) {
+    __syncthreads();  // every wave has read B_b and inv_b
+    // 3.
+    if (xw && col < W) {
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int i = 16 * iw + kq + 4 * r;
         if (i < nb) Bs[(r0 + i) * W + col] = acc[r];
       }
     }
+    // 4. Next inverse into registers.
+    const bool more = s + 1 < nblk;
+    double ipre[kInvPerThread];
+    if (more) {
+      const int bn = d.forward ? b + 1 : b - 1;
+      const double *ib = inv + d.inv0 + (int64_t)bn * kNB * kNB;
+#pragma unroll
+      for (int u = 0; u < kInvPerThread; ++u) {
+        const int e = t + u * kThreads;
+        ipre[u] = e < kInvLds ? ib[inv_src(upper, e)] : 0.0;
+      }
+    }
     __syncthreads();  // X_b visible
-    // B_rest -= op(A)[rest, b] * X_b, rest = rows after the block (lower) or
-    // before it (upper), in 16-row blocks spread over the waves. Lane l reads
-    // op(A)(rb + l%16, r0 + 4t + l/16) for t = 0..15: all 16 loads of a
-    // row-block are issued before its 16 MFMAs.
-    const int lo = d.upper ? 0 : r0 + nb, hi = d.upper ? r0 : q;
-    const int64_t si = d.trans ? d.lda : 1;  // op(A) row step
-    const int64_t sj = d.trans ? 1 : d.lda;  // op(A) column step
     double xq[16];
 #pragma unroll
     for (int t4 = 0; t4 < 16; ++t4) {
       const int j = 4 * t4 + kq;
       xq[t4] = (col < W && j < nb) ? Bs[(r0 + j) * W + col] : 0.0;
     }
-    for (int rb = lo + 16 * wave; rb < hi; rb += 64) {
-      const int64_t ar = rb + (lane & 15);
-      const bool row_ok = ar < hi;
-      const T *ap = d.a + ar * si + (int64_t)(r0 + kq) * sj;
-      double pv[16];
+    // Two register sets, each loaded one row block ahead of its MFMAs.
+    while (rb < hi) {
+      load_rows(d, rb + 16 * kWaves, hi, r0, nb, si, sj, lane, pb);
+      update_rows(Bs, pa, xq, rb, hi, nb, W, lane);
+      rb += 16 * kWaves;
+      if (rb >= hi) break;
+      load_rows(d, rb + 16 * kWaves, hi, r0, nb, si, sj, lane, pa);
+      update_rows(Bs, pb, xq, rb, hi, nb, W, lane);
+      rb += 16 * kWaves;
+    }
+    // 5.
+    if (more) {
 #pragma unroll
-      for (int t4 = 0; t4 < 16; ++t4) {
-        const int j = 4 * t4 + kq;
-        pv[t4] = (row_ok && j < nb) ? (double)ap[(int64_t)(4 * t4) * sj] : 0.0;
-      }
-      f64x4 u = {0, 0, 0, 0};
-#pragma unroll
-      for (int t4 = 0; t4 < 16; ++t4) u = mma64(pv[t4], xq[t4], u);
-      if (col < W) {
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const int i = rb + kq + 4 * r;
-          if (i < hi) Bs[i * W + col] -= u[r];
-        }
+      for (int u = 0; u < kInvPerThread; ++u) {
+        const int e = t + u * kThreads;
+        if (e < kInvLds) Is[e] = ipre[u];
       }
     }
     __syncthreads();
@@ -211,6 +343,7 @@ __global__ __launch_bounds__(kThreads) void grouped_trsm_kernel(const TrsmDesc<T
                                                                 const int2 *units, int nunits,
                                                                 int *sched, const double *inv) {
   __shared__ double Bs[kLdsDoubles];
+  __shared__ double Is[kInvLds];
   __shared__ int s_unit;
   for (;;) {
     if (threadIdx.x == 0) s_unit = atomicAdd(&sched[0], 1);
@@ -221,9 +354,9 @@ __global__ __launch_bounds__(kThreads) void grouped_trsm_kernel(const TrsmDesc<T
     const int2 w = units[u];
     const TrsmDesc<T> d = descs[uni(w.x)];
     if (d.width == 16)
-      solve_unit<T, 16>(d, uni(w.y), inv, Bs);
+      solve_unit<T, 16>(d, uni(w.y), inv, Bs, Is);
     else
-      solve_unit<T, 8>(d, uni(w.y), inv, Bs);
+      solve_unit<T, 8>(d, uni(w.y), inv, Bs, Is);
     __syncthreads();
   }
   if (threadIdx.x == 0) {
@@ -235,7 +368,43 @@ __global__ __launch_bounds__(kThreads) void grouped_trsm_kernel(const TrsmDesc<T
   }
 }
 
+// One problem passed by value: workgroup u solves right-hand sides
+// u*width .. u*width + width - 1.
+template <typename T>
+__global__ __launch_bounds__(kThreads) void single_trsm_solve(TrsmDesc<T> d, const double *inv) {
+  __shared__ double Bs[kLdsDoubles];
+  __shared__ double Is[kInvLds];
+  if (d.width == 16)
+    solve_unit<T, 16>(d, (int)blockIdx.x * 16, inv, Bs, Is);
+  else
+    solve_unit<T, 8>(d, (int)blockIdx.x * 8, inv, Bs, Is);
+}
+
 }  // namespace
+
+template <typename T>
+hipError_t launch_trsm_single(const TrsmDesc<T> &d, double *inv, hipStream_t st) {
+  if (d.q <= 0 || d.nrhs <= 0) return hipSuccess;
+  const int nblk = (d.q + kNB - 1) / kNB;
+  const int units = (d.nrhs + d.width - 1) / d.width;
+  hipLaunchKernelGGL((single_trsm_inv<T>), dim3((unsigned)nblk), dim3(kInvThreads), 0, st, d, inv);
+  hipLaunchKernelGGL((single_trsm_solve<T>), dim3((unsigned)units), dim3(kThreads), 0, st, d,
+                     (const double *)inv);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_trsm_invert(const TrsmDesc<T> &d, double *inv, hipStream_t st) {
+  if (d.q <= 0) return hipSuccess;
+  const int nblk = (d.q + kNB - 1) / kNB;
+  hipLaunchKernelGGL((single_trsm_inv<T>), dim3((unsigned)nblk), dim3(kInvThreads), 0, st, d, inv);
+  return hipGetLastError();
+}
+
+template hipError_t launch_trsm_invert<float>(const TrsmDesc<float> &, double *, hipStream_t);
+template hipError_t launch_trsm_invert<double>(const TrsmDesc<double> &, double *, hipStream_t);
+template hipError_t launch_trsm_single<float>(const TrsmDesc<float> &, double *, hipStream_t);
+template hipError_t launch_trsm_single<double>(const TrsmDesc<double> &, double *, hipStream_t);
 
 template <typename T>
 int grouped_trsm_grid() {
@@ -252,7 +421,7 @@ hipError_t launch_grouped_trsm(const TrsmDesc<T> *descs, const int2 *blocks, int
                                const int2 *units, int nunits, int grid, int *sched, double *inv,
                                hipStream_t st) {
   if (nblocks > 0)
-    hipLaunchKernelGGL((grouped_trsm_inv<T>), dim3((unsigned)nblocks), dim3(kNB), 0, st, descs,
+    hipLaunchKernelGGL((grouped_trsm_inv<T>), dim3((unsigned)nblocks), dim3(kInvThreads), 0, st, descs,
                        blocks, inv);
   if (nunits > 0) {
     if (grid > nunits) grid = nunits;

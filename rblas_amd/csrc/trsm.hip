@@ -1,10 +1,14 @@
 // Blocked triangular solve on gfx950 (level3::trsm, level3.rs:390-520):
 // op(A) X = alpha B (Left) or X op(A) = alpha B (Right), in place in B.
 //
-// rblas substitutes element by element. Here:
+// rblas substitutes element by element. Here, for an order q = m (Left) or
+// n (Right) up to 2048, the solve is the LDS-resident one of grouped_trsm.hip
+// (one launch inverting the diagonal blocks, one launch of one workgroup per
+// 16 or 8 right-hand sides, which stay in LDS through the whole
+// substitution). For larger orders:
 //   1. B := alpha B when alpha != 1 (level3.rs:403-409; every entry, so a
 //      zero alpha keeps NaN).
-//   2. trsm_diag_inv inverts every 64 x 64 diagonal block of op(A) at once,
+//   2. launch_trsm_invert inverts every 64 x 64 diagonal block of op(A) at once,
 //      in f64 for f32 and f64 operands alike (Unit: the stored diagonal is
 //      not read).
 //   3. The diagonal range is solved by recursive halving: solve the half
@@ -15,62 +19,18 @@
 //      B_b * inv_b (Right) in f64, summing only over the triangle, so a
 //      non-finite B entry reaches exactly the entries that depend on it in
 //      rblas's substitution (no 0 * NaN from structural zeros).
-#include "kernels.hpp"
+#include "grouped.hpp"
 
 namespace rblas {
 namespace {
 
 constexpr int kNB = 64;
+constexpr int64_t kDirectMaxOrder = 2048;  // q * 8 fits the LDS block
 
 // op(A)(r, c) of the stored column-major A.
 template <typename T>
 __device__ __forceinline__ T op_a(const T *a, int64_t lda, bool trans, int64_t r, int64_t c) {
   return trans ? a[c + r * lda] : a[r + c * lda];
-}
-
-// Inverse of diagonal block b of op(A) (rows/cols b*64 .. min(q, b*64+64)),
-// written column-major 64 x 64 (ld 64, f64) at inv + b*64*64; entries outside
-// the effective triangle and outside the block are zero. Thread j builds
-// column j by substitution against the unit vector e_j.
-template <typename T>
-__global__ __launch_bounds__(kNB) void trsm_diag_inv(const T *a, int64_t lda, bool trans,
-                                                     bool upper, bool unit, int64_t q,
-                                                     double *inv) {
-  __shared__ double L[kNB][kNB + 1];
-  __shared__ double X[kNB][kNB + 1];
-  const int64_t r0 = (int64_t)blockIdx.x * kNB;
-  const int nb = (int)((q - r0) < kNB ? (q - r0) : kNB);
-  const int t = threadIdx.x;
-  for (int idx = t; idx < kNB * kNB; idx += kNB) {
-    const int r = idx % kNB, c = idx / kNB;
-    double v = 0.0;
-    const bool in_tri = upper ? r <= c : r >= c;
-    if (r < nb && c < nb && in_tri)
-      v = (unit && r == c) ? 1.0 : (double)op_a(a, lda, trans, r0 + r, r0 + c);
-    L[r][c] = v;
-    X[r][c] = 0.0;
-  }
-  __syncthreads();
-  const int j = t;
-  if (j < nb) {
-    X[j][j] = 1.0 / L[j][j];
-    if (!upper) {
-      for (int i = j + 1; i < nb; ++i) {
-        double s = 0.0;
-        for (int kk = j; kk < i; ++kk) s = __builtin_fma(L[i][kk], X[kk][j], s);
-        X[i][j] = -s / L[i][i];
-      }
-    } else {
-      for (int i = j - 1; i >= 0; --i) {
-        double s = 0.0;
-        for (int kk = i + 1; kk <= j; ++kk) s = __builtin_fma(L[i][kk], X[kk][j], s);
-        X[i][j] = -s / L[i][i];
-      }
-    }
-  }
-  __syncthreads();
-  double *out = inv + (int64_t)blockIdx.x * kNB * kNB;
-  for (int idx = t; idx < kNB * kNB; idx += kNB) out[idx] = X[idx % kNB][idx / kNB];
 }
 
 // Left leaf: B[r0 + i, col] := sum_{j in tri(i)} inv[i][j] * B[r0 + j, col]
@@ -234,14 +194,39 @@ hipError_t launch_trsm(const TrsmProblem<T> &p, void *ws, hipStream_t st) {
   const int64_t m = p.m, n = p.n;
   if (m == 0 || n == 0) return hipSuccess;
   const int64_t q = p.left ? m : n;
-  const int64_t nblk = (q + kNB - 1) / kNB;
+  if (q <= kDirectMaxOrder && (p.left ? n : m) <= (int64_t)INT32_MAX - 64) {
+    // Orders up to 2048: the right-hand sides of each workgroup stay in LDS
+    // and the whole solve is two launches (grouped_trsm.hip), in Left form
+    // (a Right problem is solved on B^T with op(A)^T).
+    TrsmDesc<T> d{};
+    d.a = p.a;
+    d.b = p.b;
+    d.lda = p.lda;
+    d.ldb = p.ldb;
+    d.q = (int)q;
+    d.nrhs = (int)(p.left ? n : m);
+    d.trans = p.left ? p.trans : !p.trans;
+    d.upper = p.left ? p.upper : !p.upper;
+    d.unit = p.unit;
+    d.right = !p.left;
+    d.forward = !d.upper;
+    d.width = q <= kTrsmLdsDoubles / 16 ? 16 : 8;
+    d.alpha = p.alpha;
+    d.inv0 = 0;
+    return launch_trsm_single<T>(d, (double *)ws, st);
+  }
   double *inv = (double *)ws;
   if (p.alpha != T(1))
     hipLaunchKernelGGL((mul_kernel<T>), dim3(grid_for(m * n)), dim3(256), 0, st, m, n, p.alpha,
                        p.b, p.ldb);
-  hipLaunchKernelGGL((trsm_diag_inv<T>), dim3((unsigned)nblk), dim3(kNB), 0, st, p.a, p.lda,
-                     p.trans, p.upper, p.unit, q, inv);
-  hipError_t e = hipGetLastError();
+  TrsmDesc<T> dd{};
+  dd.a = p.a;
+  dd.lda = p.lda;
+  dd.q = (int)q;
+  dd.trans = p.trans;
+  dd.upper = p.upper;
+  dd.unit = p.unit;
+  hipError_t e = launch_trsm_invert<T>(dd, inv, st);
   if (e != hipSuccess) return e;
   // Left+lower and Right+upper substitute from index 0 up, the others down.
   Solver<T> s{p, inv, st, p.left ? !p.upper : p.upper};

@@ -349,3 +349,46 @@ def test_trsm_device_entry_and_large(gpu, side):
     s.synchronize()
     assert np.array_equal(db.download(s), bh)
     s.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("q", [1024, 1025, 2048, 2049, 2600])
+@pytest.mark.parametrize("side,uplo,trans", [(0, 1, 0), (0, 0, 1), (1, 0, 0), (1, 1, 1)])
+def test_trsm_order_switches(gpu, q, side, uplo, trans):
+    """Orders on both sides of the 16/8 right-hand-side width switch (1024)
+    and of the LDS-resident/recursive switch (2048), f32 and f64."""
+    nrhs = 37
+    m, n = (q, nrhs) if side == 0 else (nrhs, q)
+    for dtype in (np.float32, np.float64):
+        rng = np.random.default_rng(q + 10 * side + 3 * uplo + trans)
+        lda, ldb = q + 1, m + 2
+        a, opA = _tri_system(q, uplo, trans, 0, lda, dtype, rng)
+        b = rng.standard_normal((n - 1) * ldb + m).astype(dtype)
+        b0 = b.copy()
+        level3.trsm(side, uplo, trans, 0, m, n, 0.5, a, lda, b, ldb)
+        X = view(b, COL, m, n, ldb).astype(np.float64)
+        B0 = view(b0, COL, m, n, ldb).astype(np.float64)
+        ratio = _residual_ok(side, opA, X, 0.5, B0, q, dtype)
+        assert ratio <= 1.0, f"{dtype.__name__} residual over bound ({ratio:.3g})"
+        keep = np.ones(b.size, bool)
+        view(keep, COL, m, n, ldb)[:] = False
+        assert np.array_equal(b[keep], b0[keep])
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("q", [2048, 2200])
+@pytest.mark.parametrize("side", [0, 1])
+def test_trsm_exact_integers_large(gpu, q, side):
+    """The bidiagonal exact-integer system at orders taken by each path."""
+    nrhs = 20
+    m, n = (q, nrhs) if side == 0 else (nrhs, q)
+    rng = np.random.default_rng(77 + q + side)
+    S = np.diag(rng.choice([-1.0, 1.0], q))
+    off = rng.integers(-1, 2, q - 1).astype(np.float64)
+    opA = S + np.diag(off, -1)
+    X = rng.integers(-4, 5, (m, n)).astype(np.float64)
+    B = opA @ X if side == 0 else X @ opA
+    a = opA.T.reshape(-1).copy()
+    b = B.T.reshape(-1).copy()
+    level3.trsm(side, 1, 0, 0, m, n, 1.0, a, q, b, m)
+    assert np.array_equal(b.reshape(n, m).T, X)
