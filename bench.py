@@ -16,6 +16,8 @@ synthetic data of the stated shape:
   - grouped GEMM (512 problems, m, n, k log-uniform in [16, 2048], random
     transposes, U[-1,1], f32 and f64) in one persistent launch, against the
     same problems as one launch each;
+  - single trsm calls (Left / Lower / NoTrans, q = nrhs in {1024, 2048,
+    8192}, f64 and f32);
   - grouped trsm (256 lower systems, n log-uniform [32, 1024], nrhs in
     [1, 256], diag 2+|N(0,1)|, off-diagonal N(0,1)/sqrt(n), f64) against
     per-call trsm;
@@ -278,6 +280,32 @@ def grouped_trsm():
             "per_call_ms": round(ms_pc, 3), "speedup_vs_per_call": round(ms_pc / ms, 1)}
 
 
+def trsm_single():
+    """Single trsm calls on resident operands: Left / Lower / NoTrans,
+    q = nrhs in {1024, 2048, 8192}, f64 and f32 (TFLOP/s in q^2 nrhs)."""
+    from rblas_amd import level3
+    from rblas_amd.device import DeviceArray, Stream
+    rng = np.random.default_rng(11)
+    s = Stream()
+    out = {}
+    for q in (1024, 2048, 8192):
+        full = rng.standard_normal((q, q)) / np.sqrt(q)
+        np.fill_diagonal(full, 2.0 + np.abs(rng.standard_normal(q)))
+        a64 = np.tril(full).T.reshape(-1).copy()
+        b64 = rng.standard_normal(q * q)
+        del full
+        for dt, tag in ((np.float64, "f64"), (np.float32, "f32")):
+            da = DeviceArray.from_host(a64.astype(dt), s)
+            db = DeviceArray.from_host(b64.astype(dt), s)
+            steps = 5 if q == 8192 else 10
+            ms = timed(lambda: level3.trsm_dev(0, 1, 0, 0, q, q, 1.0, da, q, db, q, s), s, steps, 2)
+            out[f"{tag}_{q}"] = {"ms": round(ms, 3), "tflops": round(float(q) ** 3 / ms / 1e9, 2)}
+            da.free()
+            db.free()
+    s.close()
+    return out
+
+
 def serving(count):
     from rblas_amd import serve
     closed = serve.workload(count, 0.0)
@@ -365,6 +393,7 @@ ONLY = {
     "grouped_gemm_f64": lambda a: grouped_gemm(np.float64),
     "grouped_gemm_f32": lambda a: grouped_gemm(np.float32),
     "grouped_trsm_f64": lambda a: grouped_trsm(),
+    "trsm": lambda a: trsm_single(),
     "serve": lambda a: serving(a.serve_requests),
     "c64_gemm": lambda a: c64_gemm(),
     "sgemm_skinny": lambda a: skinny_sgemm_tflops(),
@@ -435,6 +464,8 @@ def main():
         log("grouped gemm ...")
         comp["grouped_gemm_f64"] = grouped_gemm(np.float64)
         comp["grouped_gemm_f32"] = grouped_gemm(np.float32)
+        log("trsm ...")
+        comp["trsm"] = trsm_single()
         log("grouped trsm ...")
         comp["grouped_trsm_f64"] = grouped_trsm()
         log("serving ...")

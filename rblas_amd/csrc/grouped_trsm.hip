@@ -33,6 +33,7 @@ constexpr int kLdsDoubles = kTrsmLdsDoubles;  // q * W limit (128 KiB)
 constexpr int kSub = 256;           // doubles in a 16 x 16 sub-block
 constexpr int kInvLds = 10 * kSub;  // the 10 sub-blocks of a triangular 64 x 64 block
 constexpr int kInvPerThread = (kInvLds + kThreads - 1) / kThreads;
+constexpr int kLoadBatch = 8;       // right-hand-side loads in flight per thread
 
 typedef double f64x4 __attribute__((ext_vector_type(4)));
 
@@ -60,42 +61,51 @@ __device__ __forceinline__ double lane_value(double v, int l) {
 // the block's nb x nb corner. An upper block is inverted as the lower one
 // P U P (P reverses the nb indices). Column-oriented substitution against
 // the identity, in registers: lane i holds row i, wave w columns 4w..4w+3;
-// step k divides row k by L(k, k) and subtracts L(i, k) times it from every
-// later row i. Column j only takes part from step j on (earlier steps would
-// only subtract multiples of its zeros), so each entry gets the same
-// operations, in the same order, as a row-by-row substitution of that column.
+// step k scales row k by 1 / L(k, k) and subtracts L(i, k) times it from
+// every later row i. Column j only takes part from step j on (earlier steps
+// would only subtract multiples of its zeros).
 template <typename T>
 __device__ __forceinline__ void invert_block(const TrsmDesc<T> &d, int blk, double *inv) {
   __shared__ double Ls[kNB][kNB + 1];  // Ls[k][i] = L(i, k), lower form
   const int64_t r0 = (int64_t)blk * kNB;
   const int nb = (int)min((int64_t)kNB, (int64_t)d.q - r0);
   const int t = threadIdx.x;
-  for (int idx = t; idx < kNB * kNB; idx += kInvThreads) {
+  constexpr int kFill = kNB * kNB / kInvThreads;
+  T raw[kFill];
+#pragma unroll
+  for (int u = 0; u < kFill; ++u) {  // all loads issued first, no branches
+    const int idx = t + u * kInvThreads;
     const int i = idx % kNB, k = idx / kNB;
-    double v = i == k ? 1.0 : 0.0;  // padding beyond nb: identity
-    if (i < nb && k < nb && i >= k) {
-      const int r = d.upper ? nb - 1 - i : i, c = d.upper ? nb - 1 - k : k;
-      v = (d.unit && i == k) ? 1.0 : opa(d, r0 + r, r0 + c);
-    }
-    Ls[k][i] = v;
+    const bool in = i < nb && k < nb && i > k;
+    const int64_t r = r0 + (d.upper ? nb - 1 - i : i), c = r0 + (d.upper ? nb - 1 - k : k);
+    const bool diag = i == k && i < nb && !d.unit;
+    const T *p = d.trans ? d.a + c + r * d.lda : d.a + r + c * d.lda;
+    raw[u] = *((in || diag) ? p : d.a);
+  }
+#pragma unroll
+  for (int u = 0; u < kFill; ++u) {
+    const int idx = t + u * kInvThreads;
+    const int i = idx % kNB, k = idx / kNB;
+    const bool in = i < nb && k < nb && i > k;
+    const bool diag = i == k && i < nb && !d.unit;
+    Ls[k][i] = (in || diag) ? (double)raw[u] : (i == k ? 1.0 : 0.0);  // padding: identity
   }
   __syncthreads();
   const int lane = t & 63, w = uni(t >> 6);
   const int j0 = 4 * w;
+  const double rl = 1.0 / Ls[lane][lane];
   double x[4];
 #pragma unroll
   for (int c = 0; c < 4; ++c) x[c] = lane == j0 + c ? 1.0 : 0.0;
   for (int k = j0; k < kNB; ++k) {
-    const double lkk = Ls[k][k];
     const double lik = Ls[k][lane];
+    const double rk = lane_value(rl, k);
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
       if (k >= j0 + c) {  // wave-uniform
-        const double xk = lane_value(x[c], k) / lkk;
-        if (lane == k)
-          x[c] = xk;
-        else if (lane > k)
-          x[c] = __builtin_fma(-lik, xk, x[c]);
+        const double xk = lane_value(x[c], k) * rk;
+        const double upd = __builtin_fma(-lik, xk, x[c]);
+        x[c] = lane == k ? xk : (lane > k ? upd : x[c]);
       }
     }
   }
@@ -216,20 +226,40 @@ __device__ void solve_unit(const TrsmDesc<T> &d, int c0, const double *inv, doub
   const int nc = min(W, d.nrhs - c0);
   const int qpad = (q + kNB - 1) / kNB * kNB;
   const bool upper = d.upper;
-  // Load (alpha applied to every entry, as rblas's pre-scale: 0 * NaN = NaN).
-  for (int e = t; e < qpad * W; e += kThreads) {
-    const int r = e / W, c = e % W;
-    double v = 0.0;
-    if (r < q && c < nc) {
-      v = (double)d.b[b_index(d, r, c0 + c)];
-      if (d.alpha != T(1)) v = (double)(d.alpha * (T)v);
+  // Load (alpha applied to every entry, as rblas's pre-scale: 0 * NaN = NaN),
+  // 8 loads per thread in flight at a time, without branches.
+  const int total = qpad * W;
+  for (int e0 = t; e0 < total; e0 += kLoadBatch * kThreads) {
+    T raw[kLoadBatch];
+#pragma unroll
+    for (int u = 0; u < kLoadBatch; ++u) {
+      const int e = e0 + u * kThreads;
+      const int r = e / W, c = e % W;
+      const bool ok = e < total && r < q && c < nc;
+      raw[u] = *(ok ? d.b + b_index(d, r, c0 + c) : d.b);
     }
-    Bs[e] = v;
+#pragma unroll
+    for (int u = 0; u < kLoadBatch; ++u) {
+      const int e = e0 + u * kThreads;
+      const int r = e / W, c = e % W;
+      const T v = d.alpha != T(1) ? d.alpha * raw[u] : raw[u];
+      if (e < total) Bs[e] = (r < q && c < nc) ? (double)v : 0.0;
+    }
   }
   const int nblk = qpad / kNB;
   {
     const double *ib = inv + d.inv0 + (int64_t)(d.forward ? 0 : nblk - 1) * kNB * kNB;
-    for (int e = t; e < kInvLds; e += kThreads) Is[e] = ib[inv_src(upper, e)];
+    double iv[kInvPerThread];
+#pragma unroll
+    for (int u = 0; u < kInvPerThread; ++u) {
+      const int e = t + u * kThreads;
+      iv[u] = ib[e < kInvLds ? inv_src(upper, e) : 0];
+    }
+#pragma unroll
+    for (int u = 0; u < kInvPerThread; ++u) {
+      const int e = t + u * kThreads;
+      if (e < kInvLds) Is[e] = iv[u];
+    }
   }
   __syncthreads();
   const int col = lane & 15;  // MFMA output column / Q-operand column
@@ -302,7 +332,7 @@ __device__ void solve_unit(const TrsmDesc<T> &d, int c0, const double *inv, doub
 #pragma unroll
       for (int u = 0; u < kInvPerThread; ++u) {
         const int e = t + u * kThreads;
-        ipre[u] = e < kInvLds ? ib[inv_src(upper, e)] : 0.0;
+        ipre[u] = ib[e < kInvLds ? inv_src(upper, e) : 0];
       }
     }
     __syncthreads();  // X_b visible

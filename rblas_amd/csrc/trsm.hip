@@ -44,23 +44,38 @@ __global__ __launch_bounds__(256) void trsm_apply_left(const double *inv, bool u
   const int t = threadIdx.x, i = t % kNB, c0 = t / kNB;
   const int64_t col0 = (int64_t)blockIdx.x * kNB;
   const int ncol = (int)((n - col0) < kNB ? (n - col0) : kNB);
-  for (int idx = t; idx < kNB * kNB; idx += 256) invT[idx / kNB][idx % kNB] = inv[idx];
+  // Every load issued before any is used, without branches (an entry outside
+  // the block reads b[0] and is replaced by 0).
+  double iv[16];
+  T raw[16];
+#pragma unroll
+  for (int r = 0; r < 16; ++r) iv[r] = inv[t + 256 * r];
+#pragma unroll
   for (int r = 0; r < 16; ++r) {
     const int c = c0 + 4 * r;
-    Bs[c][i] = (c < ncol && i < nb) ? (double)b[i + (col0 + c) * ldb] : 0.0;
+    const bool ok = c < ncol && i < nb;
+    raw[r] = *(ok ? b + i + (col0 + c) * ldb : b);
+  }
+#pragma unroll
+  for (int r = 0; r < 16; ++r) {
+    const int idx = t + 256 * r;
+    invT[idx / kNB][idx % kNB] = iv[r];
+    const int c = c0 + 4 * r;
+    Bs[c][i] = (c < ncol && i < nb) ? (double)raw[r] : 0.0;
   }
   __syncthreads();
   double acc[16];
 #pragma unroll
   for (int r = 0; r < 16; ++r) acc[r] = 0.0;
-  for (int j = 0; j < nb; ++j) {
+  // Row i sums over its triangle only: j = 0..i (lower) or nb-1 down to i
+  // (upper). j is the same in every lane still running, so Bs reads
+  // broadcast; lanes leave the loop when their triangle ends.
+  const int cnt = i < nb ? (upper ? nb - i : i + 1) : 0;
+  for (int s = 0; s < cnt; ++s) {
+    const int j = upper ? nb - 1 - s : s;
     const double w = invT[j][i];
-    const bool on = upper ? j >= i : j <= i;
 #pragma unroll
-    for (int r = 0; r < 16; ++r) {
-      const double v = __builtin_fma(w, Bs[c0 + 4 * r][j], acc[r]);
-      acc[r] = on ? v : acc[r];
-    }
+    for (int r = 0; r < 16; ++r) acc[r] = __builtin_fma(w, Bs[c0 + 4 * r][j], acc[r]);
   }
   if (i < nb) {
 #pragma unroll
@@ -72,39 +87,57 @@ __global__ __launch_bounds__(256) void trsm_apply_left(const double *inv, bool u
 }
 
 // Right leaf: B[row, r0 + j] := sum_{p in tri(j)} B[row, r0 + p] * inv[p][j]
-// for the 64 rows of this workgroup. Lane row i = t % 64, columns
-// j = t / 64 + 4 r (uniform across a wavefront, so inv reads broadcast).
+// for the 64 rows of this workgroup. Loads and stores use lane = row
+// (coalesced on column-major B); the sums use lane = column j = t % 64 and
+// rows t / 64 + 4 r, so each lane's triangle (p = 0..j upper, nb-1 down to j
+// lower) is its own loop trip count, p is the same in every lane still
+// running and the row reads broadcast.
 template <typename T>
 __global__ __launch_bounds__(256) void trsm_apply_right(const double *inv, bool upper, int nb,
                                                         int64_t m, T *b, int64_t ldb) {
-  __shared__ double inv_s[kNB][kNB];  // inv_s[j][p] = inv(p, j)
-  __shared__ double Bs[kNB][kNB];     // Bs[p][i] = B(row0 + i, r0 + p)
+  __shared__ double inv_t[kNB][kNB + 1];  // inv_t[p][j] = inv(p, j)
+  __shared__ double Bs[kNB][kNB + 1];     // Bs[p][i] = B(row0 + i, r0 + p)
   const int t = threadIdx.x, i = t % kNB, c0 = t / kNB;
   const int64_t row0 = (int64_t)blockIdx.x * kNB;
   const bool row_ok = row0 + i < m;
-  for (int idx = t; idx < kNB * kNB; idx += 256) inv_s[idx / kNB][idx % kNB] = inv[idx];
+  double iv[16];
+  T raw[16];
+#pragma unroll
+  for (int r = 0; r < 16; ++r) iv[r] = inv[t + 256 * r];
+#pragma unroll
   for (int r = 0; r < 16; ++r) {
     const int p = c0 + 4 * r;
-    Bs[p][i] = (row_ok && p < nb) ? (double)b[row0 + i + p * ldb] : 0.0;
+    const bool ok = row_ok && p < nb;
+    raw[r] = *(ok ? b + row0 + i + p * ldb : b);
+  }
+#pragma unroll
+  for (int r = 0; r < 16; ++r) {
+    const int idx = t + 256 * r;
+    inv_t[idx % kNB][idx / kNB] = iv[r];
+    const int p = c0 + 4 * r;
+    Bs[p][i] = (row_ok && p < nb) ? (double)raw[r] : 0.0;
   }
   __syncthreads();
+  const int j = t % kNB, rw = t / kNB;  // column j; rows rw + 4 r
   double acc[16];
 #pragma unroll
   for (int r = 0; r < 16; ++r) acc[r] = 0.0;
-  for (int p = 0; p < nb; ++p) {
-    const double v = Bs[p][i];
+  const int cnt = j < nb ? (upper ? j + 1 : nb - j) : 0;
+  for (int s = 0; s < cnt; ++s) {
+    const int p = upper ? s : nb - 1 - s;
+    const double w = inv_t[p][j];
 #pragma unroll
-    for (int r = 0; r < 16; ++r) {
-      const int j = c0 + 4 * r;
-      const bool on = upper ? p <= j : p >= j;
-      if (on) acc[r] = __builtin_fma(v, inv_s[j][p], acc[r]);
-    }
+    for (int r = 0; r < 16; ++r) acc[r] = __builtin_fma(Bs[p][rw + 4 * r], w, acc[r]);
   }
+  __syncthreads();  // every lane has read Bs
+#pragma unroll
+  for (int r = 0; r < 16; ++r) Bs[j][rw + 4 * r] = acc[r];
+  __syncthreads();
   if (row_ok) {
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
-      const int j = c0 + 4 * r;
-      if (j < nb) b[row0 + i + j * ldb] = (T)acc[r];
+      const int p = c0 + 4 * r;
+      if (p < nb) b[row0 + i + p * ldb] = (T)Bs[p][i];
     }
   }
 }
