@@ -360,31 +360,34 @@ __global__ __launch_bounds__(kThreads) void split_gemm_kernel(SplitArgs p) {
   }
 
   // Element r of lane l in block (i, j): C row m0+wm+32i+l%32, column
-  // n0+wn+32j+8(r/4)+4(l/32)+r%4. Each block's C values are loaded together
-  // before any of them is stored (one memory round trip per block).
-  const bool read_c = p.beta != 0.f;
+  // n0+wn+32j+8(r/4)+4(l/32)+r%4. The C loads of a block row (4 blocks) are
+  // issued together without branches (an element outside C reads C[0]
+  // instead and is not stored): one memory round trip per block row.
+  const bool read_c = p.beta != 0.f;  // uniform
   const int r32 = g.lane & 31, hi = g.lane >> 5;
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
     const int gi = m0 + g.wm + 32 * i + r32;
-    if (gi >= p.m) continue;
+    float cv[4][16];
+    if (read_c) {
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      float cv[16];
+      for (int j = 0; j < 4; ++j)
 #pragma unroll
-      for (int r = 0; r < 16; ++r) {
-        const int gj = n0 + g.wn + 32 * j + 8 * (r >> 2) + 4 * hi + (r & 3);
-        cv[r] = (read_c && gj < p.n) ? p.c[gi + (int64_t)gj * p.ldc] : 0.f;
-      }
-#pragma unroll
-      for (int r = 0; r < 16; ++r) {
-        const int gj = n0 + g.wn + 32 * j + 8 * (r >> 2) + 4 * hi + (r & 3);
-        if (gj >= p.n) continue;
-        const float v = g.acc[i][j][r];
-        p.c[gi + (int64_t)gj * p.ldc] =
-            read_c ? __builtin_fmaf(p.alpha, v, p.beta * cv[r]) : p.alpha * v;
-      }
+        for (int r = 0; r < 16; ++r) {
+          const int gj = n0 + g.wn + 32 * j + 8 * (r >> 2) + 4 * hi + (r & 3);
+          const bool in = gi < p.m && gj < p.n;
+          cv[j][r] = *(in ? p.c + gi + (int64_t)gj * p.ldc : p.c);
+        }
     }
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const int gj = n0 + g.wn + 32 * j + 8 * (r >> 2) + 4 * hi + (r & 3);
+        const float v = g.acc[i][j][r];
+        const float out = read_c ? __builtin_fmaf(p.alpha, v, p.beta * cv[j][r]) : p.alpha * v;
+        if (gi < p.m && gj < p.n) p.c[gi + (int64_t)gj * p.ldc] = out;
+      }
   }
 }
 

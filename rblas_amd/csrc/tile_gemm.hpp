@@ -434,34 +434,50 @@ struct TileGemm {
   }
 
   // C := alpha * acc + beta * C on the tile at (m0, n0); beta == 0 never
-  // reads C; tri 1 / 2 keeps only the upper / lower triangle. Each block's C
-  // values are loaded together before any of them is stored.
+  // reads C; tri 1 / 2 keeps only the upper / lower triangle. The C loads of
+  // kEpiRows block rows are issued together, without branches (an element
+  // outside the matrix or the triangle reads C[0] instead and is not
+  // stored), so a tile waits on memory FM / kEpiRows times, not once per
+  // element.
+  static constexpr int kEpiRows = FM * FN * 4 <= 32 ? FM : FM / 2;
+
+  __device__ __forceinline__ bool c_in(const GemmProblem<T> &p, int gi, int gj) const {
+    return gi < (int)p.m && gj < (int)p.n &&
+           !((p.tri == 1 && gi > gj) || (p.tri == 2 && gi < gj));
+  }
+
   __device__ __forceinline__ void epilogue(const GemmProblem<T> &p, int m0, int n0) const {
-    const int M = (int)p.m, N = (int)p.n;
     const T alpha = p.alpha, beta = p.beta;
-    const bool read_c = beta != T(0);
+    const bool read_c = beta != T(0);  // uniform
 #pragma unroll
-    for (int i = 0; i < FM; ++i) {
-      const int gi = m0 + row_of(i);
-      if (gi >= M) continue;
+    for (int i0 = 0; i0 < FM; i0 += kEpiRows) {
+      T cv[kEpiRows][FN][4];
+      if (read_c) {
 #pragma unroll
-      for (int j = 0; j < FN; ++j) {
-        T cv[4];
+        for (int ii = 0; ii < kEpiRows; ++ii) {
+          const int gi = m0 + row_of(i0 + ii);
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const int gj = n0 + col_of(j, r);
-          const bool in = gj < N && !((p.tri == 1 && gi > gj) || (p.tri == 2 && gi < gj));
-          cv[r] = (read_c && in) ? p.c[gi + (int64_t)gj * p.ldc] : T(0);
+          for (int j = 0; j < FN; ++j)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+              const int gj = n0 + col_of(j, r);
+              const T *q = c_in(p, gi, gj) ? p.c + gi + (int64_t)gj * p.ldc : p.c;
+              cv[ii][j][r] = *q;
+            }
         }
+      }
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const int gj = n0 + col_of(j, r);
-          if (gj >= N) continue;
-          if ((p.tri == 1 && gi > gj) || (p.tri == 2 && gi < gj)) continue;
-          const T v = acc[i][j][r];
-          p.c[gi + (int64_t)gj * p.ldc] =
-              read_c ? __builtin_fma(alpha, v, beta * cv[r]) : alpha * v;
-        }
+      for (int ii = 0; ii < kEpiRows; ++ii) {
+        const int gi = m0 + row_of(i0 + ii);
+#pragma unroll
+        for (int j = 0; j < FN; ++j)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const int gj = n0 + col_of(j, r);
+            const T v = acc[i0 + ii][j][r];
+            const T out = read_c ? __builtin_fma(alpha, v, beta * cv[ii][j][r]) : alpha * v;
+            if (c_in(p, gi, gj)) p.c[gi + (int64_t)gj * p.ldc] = out;
+          }
       }
     }
   }
