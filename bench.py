@@ -24,7 +24,10 @@ synthetic data of the stated shape:
   - serving: 10k requests (60% gemm, 25% trsm, 15% complex gemm), Poisson
     at 70% of the measured closed-loop capacity, 200 GB admission cap:
     p50/p99 latency and TFLOP/s, continuous batching vs plain per-call;
-  - C64 gemm (4M) 4096^3 and Level-1 bandwidth.
+  - DGEMM and SGEMM 8192^3 in all 8 Layout x Transpose variants;
+  - the small SGEMM (col-major NN 256^3, U[-1,1] seed 0, beta 0.5): device
+    time per call and host-slice call time;
+  - C64 gemm (4M) 4096^3 and Level-1 bandwidth (f64 and f32).
 `roofline` prices the DGEMM kernel against MI355X's FP64 MFMA peak, with HBM
 traffic per launch from the committed rocprofv3 PMC summary; `cpu_baseline`
 times the oracle (rblas's CPU algorithm restated in C++) on a column slice
@@ -177,6 +180,59 @@ def precision_case(case, rng_seed=3):
     return {k: float(f"{v:.4g}") for k, v in out.items()}
 
 
+def gemm_variants(dtype, n=8192, steps=5):
+    """All 8 Layout x Transpose variants at n^3 on one set of resident
+    operands (stored ld = n + 7 fits every variant of a square problem)."""
+    from rblas_amd import level3, workloads
+    from rblas_amd.device import DeviceArray, Stream
+    ld = n + 7
+    la = (n - 1) * ld + n
+    s = Stream()
+    da, db, dc = (DeviceArray.from_host(workloads.normal(1, i * la, la, dtype), s)
+                  for i in range(3))
+    out = {}
+    for layout, lname in ((1, "col"), (0, "row")):
+        for ta in (0, 1):
+            for tb in (0, 1):
+                ms = timed(lambda: level3.gemm_dev(layout, ta, tb, n, n, n, 1.5, da, ld, db, ld,
+                                                   -0.25, dc, ld, s), s, steps, 1)
+                out[f"{lname}_{'NT'[ta]}{'NT'[tb]}"] = round(2.0 * n ** 3 / ms / 1e9, 2)
+    for x in (da, db, dc):
+        x.free()
+    s.close()
+    return out
+
+
+def sgemm_256(calls=200):
+    """The small plain SGEMM: col-major NN 256^3, f32, U[-1,1] seed 0 drawn A,
+    B, C, alpha 1, beta 0.5, ld 256. Latency-bound: reports the device time
+    per call (resident operands, back-to-back calls) and the host-slice call
+    time (copies in and out, synchronous)."""
+    from rblas_amd import level3, workloads
+    from rblas_amd.device import DeviceArray, Stream
+    n = 256
+    a = workloads.uniform(0, 0, n * n, np.float32)
+    b = workloads.uniform(0, n * n, n * n, np.float32)
+    c = workloads.uniform(0, 2 * n * n, n * n, np.float32)
+    s = Stream()
+    da, db, dc = (DeviceArray.from_host(x, s) for x in (a, b, c))
+    ms = timed(lambda: level3.gemm_dev(1, 0, 0, n, n, n, 1.0, da, n, db, n, 0.5, dc, n, s), s,
+               calls, 10)
+    for x in (da, db, dc):
+        x.free()
+    s.close()
+    ch = c.copy()
+    for _ in range(5):
+        level3.gemm(1, 0, 0, n, n, n, 1.0, a, n, b, n, 0.5, ch, n)
+    t0 = time.perf_counter()
+    for _ in range(50):
+        level3.gemm(1, 0, 0, n, n, n, 1.0, a, n, b, n, 0.5, ch, n)
+    host_us = (time.perf_counter() - t0) / 50 * 1e6
+    return {"device_us_per_call": round(ms * 1e3, 2),
+            "device_tflops": round(2.0 * n ** 3 / ms / 1e9, 3),
+            "host_call_us": round(host_us, 1)}
+
+
 def skinny_sgemm_tflops(steps=5):
     """m = n = 1024, k = 65536 SGEMM (the precision study's case c)."""
     from rblas_amd import level3, workloads
@@ -254,7 +310,7 @@ def grouped_trsm():
     from rblas_amd.device import DeviceArray, Stream
     rng = np.random.default_rng(5)
     s = Stream()
-    probs, flops = [], 0.0
+    probs, flops, ideal_s = [], 0.0, 0.0
     for _ in range(256):
         q = int(np.exp(rng.uniform(np.log(32), np.log(1025))))
         nr = int(rng.integers(1, 257))
@@ -264,6 +320,9 @@ def grouped_trsm():
         b = DeviceArray.from_host(rng.standard_normal(q * nr), s)
         probs.append(grouped.TrsmProblem(0, 1, 0, 0, q, nr, 1.0, a, q, b, q))
         flops += float(q) * q * nr
+        # Per-system roofline: max(q^2 nrhs / FP64 peak, (q^2/2 + 2 q nrhs) 8 B / HBM).
+        ideal_s += max(float(q) * q * nr / (FP64_PEAK * 1e12),
+                       (q * q / 2.0 + 2.0 * q * nr) * 8 / (HBM_PEAK_GBS * 1e9))
     plan = grouped.GroupedTrsm(probs, np.float64)
     ms = timed(lambda: plan.run(s), s, 10, 2)
 
@@ -277,7 +336,8 @@ def grouped_trsm():
         p.b.free()
     s.close()
     return {"tflops": round(flops / ms / 1e9, 3), "ms": round(ms, 3),
-            "per_call_ms": round(ms_pc, 3), "speedup_vs_per_call": round(ms_pc / ms, 1)}
+            "per_call_ms": round(ms_pc, 3), "speedup_vs_per_call": round(ms_pc / ms, 1),
+            "roofline_frac": round(ideal_s * 1e3 / ms, 3)}
 
 
 def trsm_single():
@@ -341,20 +401,25 @@ def c64_gemm(n=4096):
 
 
 def level1_bw(n=1 << 26):
+    """dot / axpy / nrm2 at n = 2^26, unit stride, f64 and f32: GB/s of
+    algorithmic bytes (2ns, 3ns, ns)."""
     from rblas_amd import level1, workloads
     from rblas_amd.device import DeviceArray, Stream
     s = Stream()
-    x = DeviceArray.from_host(workloads.uniform(3, 0, n, np.float64), s)
-    y = DeviceArray.from_host(workloads.uniform(3, n, n, np.float64), s)
-    r = DeviceArray(1, np.float64)
     out = {}
-    for name, fn, nbytes in (("ddot", lambda: level1.dot_dev(n, x, 1, y, 1, r, s), 16 * n),
-                             ("daxpy", lambda: level1.axpy_dev(n, 1e-3, x, 1, y, 1, s), 24 * n),
-                             ("dnrm2", lambda: level1.nrm2_dev(n, x, 1, r, s), 8 * n)):
-        ms = timed(fn, s, 20, 3)
-        out[name + "_gbs"] = round(nbytes / ms / 1e6, 1)
-    for a in (x, y, r):
-        a.free()
+    for dt, pre in ((np.float64, "d"), (np.float32, "s")):
+        sz = np.dtype(dt).itemsize
+        x = DeviceArray.from_host(workloads.uniform(3, 0, n, dt), s)
+        y = DeviceArray.from_host(workloads.uniform(3, n, n, dt), s)
+        r = DeviceArray(1, dt)
+        for name, fn, nbytes in (("dot", lambda: level1.dot_dev(n, x, 1, y, 1, r, s), 2 * n * sz),
+                                 ("axpy", lambda: level1.axpy_dev(n, 1e-3, x, 1, y, 1, s),
+                                  3 * n * sz),
+                                 ("nrm2", lambda: level1.nrm2_dev(n, x, 1, r, s), n * sz)):
+            ms = timed(fn, s, 20, 3)
+            out[pre + name + "_gbs"] = round(nbytes / ms / 1e6, 1)
+        for a in (x, y, r):
+            a.free()
     s.close()
     return out
 
@@ -398,6 +463,9 @@ ONLY = {
     "c64_gemm": lambda a: c64_gemm(),
     "sgemm_skinny": lambda a: skinny_sgemm_tflops(),
     "level1": lambda a: level1_bw(),
+    "dgemm_variants": lambda a: gemm_variants(np.float64),
+    "sgemm_variants": lambda a: gemm_variants(np.float32),
+    "sgemm_256": lambda a: sgemm_256(),
 }
 
 
@@ -461,6 +529,10 @@ def main():
         log("precision ...")
         comp["precision_err_over_bound"] = {c: precision_case(c) for c in ("a", "b", "c")}
         comp["sgemm_skinny_1024x1024x65536_tflops"] = skinny_sgemm_tflops()
+        log("variants ...")
+        comp["dgemm_8192_variants_tflops"] = gemm_variants(np.float64)
+        comp["sgemm_8192_variants_tflops"] = gemm_variants(np.float32)
+        comp["sgemm_256"] = sgemm_256()
         log("grouped gemm ...")
         comp["grouped_gemm_f64"] = grouped_gemm(np.float64)
         comp["grouped_gemm_f32"] = grouped_gemm(np.float32)
@@ -473,7 +545,7 @@ def main():
         comp["serve_p99_ms"] = comp["serve"]["batched"]["p99_ms"]
         log("c64 / level-1 ...")
         comp["c64_gemm_4096"] = c64_gemm()
-        comp["level1_f64"] = level1_bw()
+        comp["level1"] = level1_bw()
         out["components"] = comp
     if not args.no_cpu_baseline:
         log("cpu baseline ...")

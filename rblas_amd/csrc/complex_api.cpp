@@ -63,10 +63,6 @@ Check cgemm_validate(int32_t ta, int32_t tb, size_t m, size_t n, size_t k, size_
   return Check{};
 }
 
-size_t cext(size_t rows, size_t cols, size_t ld) {
-  return rows == 0 || cols == 0 ? 0 : (cols - 1) * ld + rows;
-}
-
 template <typename T>
 int32_t cgemm_host(int32_t ta, int32_t tb, size_t m, size_t n, size_t k, T ar, T ai, const T *a,
                    size_t a_len, size_t lda, const T *b, size_t b_len, size_t ldb, T br, T bi,
@@ -75,26 +71,30 @@ int32_t cgemm_host(int32_t ta, int32_t tb, size_t m, size_t n, size_t k, T ar, T
   if (!chk.ok()) return fail(chk, nullptr);
   if (m == 0 || n == 0) return RBLAS_OK;
   const bool at = is_transposed(ta), bt = is_transposed(tb);
-  const size_t ea = cext(at ? k : m, at ? m : k, lda), eb = cext(bt ? n : k, bt ? k : n, ldb);
-  const size_t ec = cext(m, n, ldc);
   CtxLease lease;
   if (!lease.get()) return hip_fail(lease.status(), "stream create");
   Ctx &ctx = *lease.get();
   const size_t cs = 2 * sizeof(T);
-  RBLAS_HIP_TRY(ctx.ensure(0, ea * cs), "device buffer A");
-  RBLAS_HIP_TRY(ctx.ensure(1, eb * cs), "device buffer B");
-  RBLAS_HIP_TRY(ctx.ensure(2, ec * cs), "device buffer C");
+  HostXfer x(ctx, cs, (m * k + k * n + m * n) * cs);
+  Staged sa, sb, sc;
+  sa.plan(at ? k : m, at ? m : k, lda, x.pinned());
+  sb.plan(bt ? n : k, bt ? k : n, ldb, x.pinned());
+  sc.plan(m, n, ldc, x.pinned());
+  RBLAS_HIP_TRY(ctx.ensure(0, sa.elems * cs), "device buffer A");
+  RBLAS_HIP_TRY(ctx.ensure(1, sb.elems * cs), "device buffer B");
+  RBLAS_HIP_TRY(ctx.ensure(2, sc.elems * cs), "device buffer C");
   RBLAS_HIP_TRY(ctx.ensure(3, cgemm_workspace_elems(m, n, k) * sizeof(T)), "device workspace");
+  RBLAS_HIP_TRY(x.reserve(sa, sb, sc), "pinned staging buffer");
   hipStream_t st = ctx.stream;
-  if (ea) RBLAS_HIP_TRY(hipMemcpyAsync(ctx.buf[0], a, ea * cs, hipMemcpyHostToDevice, st), "copy A");
-  if (eb) RBLAS_HIP_TRY(hipMemcpyAsync(ctx.buf[1], b, eb * cs, hipMemcpyHostToDevice, st), "copy B");
-  RBLAS_HIP_TRY(hipMemcpyAsync(ctx.buf[2], c, ec * cs, hipMemcpyHostToDevice, st), "copy C");
+  RBLAS_HIP_TRY(x.in(sa, a, ctx.buf[0]), "copy A");
+  RBLAS_HIP_TRY(x.in(sb, b, ctx.buf[1]), "copy B");
+  RBLAS_HIP_TRY(x.in(sc, c, ctx.buf[2]), "copy C");
   CgemmProblem<T> p{ta, tb, (int64_t)m, (int64_t)n, (int64_t)k, ar, ai, br, bi,
-                    (const T *)ctx.buf[0], (int64_t)lda, (const T *)ctx.buf[1], (int64_t)ldb,
-                    (T *)ctx.buf[2], (int64_t)ldc};
+                    (const T *)ctx.buf[0], (int64_t)sa.dev_ld, (const T *)ctx.buf[1],
+                    (int64_t)sb.dev_ld, (T *)ctx.buf[2], (int64_t)sc.dev_ld};
   RBLAS_HIP_TRY(launch_cgemm<T>(p, (T *)ctx.buf[3], st), "complex gemm launch");
-  RBLAS_HIP_TRY(hipMemcpyAsync(c, ctx.buf[2], ec * cs, hipMemcpyDeviceToHost, st), "copy C back");
-  RBLAS_HIP_TRY(hipStreamSynchronize(st), "complex gemm");
+  RBLAS_HIP_TRY(x.out(sc, ctx.buf[2], c), "copy C back");
+  RBLAS_HIP_TRY(x.finish(), "complex gemm");
   return RBLAS_OK;
 }
 

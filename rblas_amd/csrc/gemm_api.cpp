@@ -74,27 +74,30 @@ int32_t gemm_host(const ColMajorCall<T> &cc, int planes = 0) {
   if (!lease.get()) return hip_fail(lease.status(), "stream create");
   Ctx &ctx = *lease.get();
   const bool need_ab = cc.k != 0 && cc.alpha != T(0);
+  const size_t total = (need_ab ? cc.m * cc.k + cc.k * cc.n : 0) + cc.m * cc.n;
+  HostXfer x(ctx, sizeof(T), total * sizeof(T));
   Staged sa, sb, sc;
   if (need_ab) {
-    sa.plan(cc.at ? cc.k : cc.m, cc.at ? cc.m : cc.k, cc.lda);
-    sb.plan(cc.bt ? cc.n : cc.k, cc.bt ? cc.k : cc.n, cc.ldb);
+    sa.plan(cc.at ? cc.k : cc.m, cc.at ? cc.m : cc.k, cc.lda, x.pinned());
+    sb.plan(cc.bt ? cc.n : cc.k, cc.bt ? cc.k : cc.n, cc.ldb, x.pinned());
   }
-  sc.plan(cc.m, cc.n, cc.ldc);
+  sc.plan(cc.m, cc.n, cc.ldc, x.pinned());
   RBLAS_HIP_TRY(ctx.ensure(0, sa.elems * sizeof(T)), "device buffer A");
   RBLAS_HIP_TRY(ctx.ensure(1, sb.elems * sizeof(T)), "device buffer B");
   RBLAS_HIP_TRY(ctx.ensure(2, sc.elems * sizeof(T)), "device buffer C");
   RBLAS_HIP_TRY(ctx.ensure(3, workspace_bytes(cc, planes)), "device workspace");
+  RBLAS_HIP_TRY(x.reserve(sa, sb, sc), "pinned staging buffer");
   T *da = (T *)ctx.buf[0], *db = (T *)ctx.buf[1], *dc = (T *)ctx.buf[2];
   hipStream_t st = ctx.stream;
   if (need_ab) {
-    RBLAS_HIP_TRY(stage_in(sa, cc.a, da, st), "copy A to device");
-    RBLAS_HIP_TRY(stage_in(sb, cc.b, db, st), "copy B to device");
+    RBLAS_HIP_TRY(x.in(sa, cc.a, da), "copy A to device");
+    RBLAS_HIP_TRY(x.in(sb, cc.b, db), "copy B to device");
   }
-  RBLAS_HIP_TRY(stage_in(sc, (const T *)cc.c, dc, st), "copy C to device");
+  RBLAS_HIP_TRY(x.in(sc, cc.c, dc), "copy C to device");
   RBLAS_HIP_TRY(enqueue(cc, da, sa.dev_ld, db, sb.dev_ld, dc, sc.dev_ld, planes, ctx.buf[3], st),
                 "gemm launch");
-  RBLAS_HIP_TRY(stage_out(sc, dc, cc.c, st), "copy C to host");
-  RBLAS_HIP_TRY(hipStreamSynchronize(st), "gemm");
+  RBLAS_HIP_TRY(x.out(sc, dc, cc.c), "copy C to host");
+  RBLAS_HIP_TRY(x.finish(), "gemm");
   return RBLAS_OK;
 }
 

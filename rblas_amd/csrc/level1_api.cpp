@@ -59,8 +59,6 @@ Check nrm2_check(size_t n, size_t x_len, size_t incx) {
   return strided_check("nrm2", 'x', n, incx, x_len);
 }
 
-size_t extent(size_t n, size_t inc) { return n == 0 ? 0 : (n - 1) * inc + 1; }
-
 template <typename T>
 int32_t dot_host(size_t n, const T *x, size_t x_len, size_t incx, const T *y, size_t y_len,
                  size_t incy, T *result) {
@@ -74,19 +72,24 @@ int32_t dot_host(size_t n, const T *x, size_t x_len, size_t incx, const T *y, si
   CtxLease lease;
   if (!lease.get()) return hip_fail(lease.status(), "stream create");
   Ctx &ctx = *lease.get();
-  const size_t ex = extent(n, incx), ey = extent(n, incy);
-  RBLAS_HIP_TRY(ctx.ensure(0, ex * sizeof(T)), "device buffer x");
-  RBLAS_HIP_TRY(ctx.ensure(1, ey * sizeof(T)), "device buffer y");
+  // A vector of stride inc is staged as a 1 x n operand with ld = inc.
+  HostXfer xf(ctx, sizeof(T), 2 * n * sizeof(T));
+  Staged sx, sy;
+  sx.plan(1, n, incx, xf.pinned());
+  sy.plan(1, n, incy, xf.pinned());
+  RBLAS_HIP_TRY(ctx.ensure(0, sx.elems * sizeof(T)), "device buffer x");
+  RBLAS_HIP_TRY(ctx.ensure(1, sy.elems * sizeof(T)), "device buffer y");
   RBLAS_HIP_TRY(ctx.ensure(3, level1_scratch_bytes() + 64), "device scratch");
+  RBLAS_HIP_TRY(xf.reserve(sx, sy), "pinned staging buffer");
   hipStream_t st = ctx.stream;
-  RBLAS_HIP_TRY(hipMemcpyAsync(ctx.buf[0], x, ex * sizeof(T), hipMemcpyHostToDevice, st), "copy x");
-  RBLAS_HIP_TRY(hipMemcpyAsync(ctx.buf[1], y, ey * sizeof(T), hipMemcpyHostToDevice, st), "copy y");
+  RBLAS_HIP_TRY(xf.in(sx, x, ctx.buf[0]), "copy x");
+  RBLAS_HIP_TRY(xf.in(sy, y, ctx.buf[1]), "copy y");
   T *dres = (T *)((char *)ctx.buf[3] + level1_scratch_bytes());
-  RBLAS_HIP_TRY(launch_dot<T>((int64_t)n, (const T *)ctx.buf[0], (int64_t)incx,
-                              (const T *)ctx.buf[1], (int64_t)incy, dres, ctx.buf[3], st),
+  RBLAS_HIP_TRY(launch_dot<T>((int64_t)n, (const T *)ctx.buf[0], (int64_t)sx.dev_ld,
+                              (const T *)ctx.buf[1], (int64_t)sy.dev_ld, dres, ctx.buf[3], st),
                 "dot launch");
   RBLAS_HIP_TRY(hipMemcpyAsync(result, dres, sizeof(T), hipMemcpyDeviceToHost, st), "copy result");
-  RBLAS_HIP_TRY(hipStreamSynchronize(st), "dot");
+  RBLAS_HIP_TRY(xf.finish(), "dot");
   return RBLAS_OK;
 }
 
@@ -102,17 +105,20 @@ int32_t nrm2_host(size_t n, const T *x, size_t x_len, size_t incx, T *result) {
   CtxLease lease;
   if (!lease.get()) return hip_fail(lease.status(), "stream create");
   Ctx &ctx = *lease.get();
-  const size_t ex = extent(n, incx);
-  RBLAS_HIP_TRY(ctx.ensure(0, ex * sizeof(T)), "device buffer x");
+  HostXfer xf(ctx, sizeof(T), n * sizeof(T));
+  Staged sx;
+  sx.plan(1, n, incx, xf.pinned());
+  RBLAS_HIP_TRY(ctx.ensure(0, sx.elems * sizeof(T)), "device buffer x");
   RBLAS_HIP_TRY(ctx.ensure(3, level1_scratch_bytes() + 64), "device scratch");
+  RBLAS_HIP_TRY(xf.reserve(sx), "pinned staging buffer");
   hipStream_t st = ctx.stream;
-  RBLAS_HIP_TRY(hipMemcpyAsync(ctx.buf[0], x, ex * sizeof(T), hipMemcpyHostToDevice, st), "copy x");
+  RBLAS_HIP_TRY(xf.in(sx, x, ctx.buf[0]), "copy x");
   T *dres = (T *)((char *)ctx.buf[3] + level1_scratch_bytes());
-  RBLAS_HIP_TRY(launch_nrm2<T>((int64_t)n, (const T *)ctx.buf[0], (int64_t)incx, dres, ctx.buf[3],
-                               st),
+  RBLAS_HIP_TRY(launch_nrm2<T>((int64_t)n, (const T *)ctx.buf[0], (int64_t)sx.dev_ld, dres,
+                               ctx.buf[3], st),
                 "nrm2 launch");
   RBLAS_HIP_TRY(hipMemcpyAsync(result, dres, sizeof(T), hipMemcpyDeviceToHost, st), "copy result");
-  RBLAS_HIP_TRY(hipStreamSynchronize(st), "nrm2");
+  RBLAS_HIP_TRY(xf.finish(), "nrm2");
   return RBLAS_OK;
 }
 
@@ -126,17 +132,21 @@ int32_t axpy_host(size_t n, T alpha, const T *x, size_t x_len, size_t incx, T *y
   CtxLease lease;
   if (!lease.get()) return hip_fail(lease.status(), "stream create");
   Ctx &ctx = *lease.get();
-  const size_t ex = extent(n, incx), ey = extent(n, incy);
-  RBLAS_HIP_TRY(ctx.ensure(0, ex * sizeof(T)), "device buffer x");
-  RBLAS_HIP_TRY(ctx.ensure(1, ey * sizeof(T)), "device buffer y");
+  HostXfer xf(ctx, sizeof(T), 2 * n * sizeof(T));
+  Staged sx, sy;
+  sx.plan(1, n, incx, xf.pinned());
+  sy.plan(1, n, incy, xf.pinned());
+  RBLAS_HIP_TRY(ctx.ensure(0, sx.elems * sizeof(T)), "device buffer x");
+  RBLAS_HIP_TRY(ctx.ensure(1, sy.elems * sizeof(T)), "device buffer y");
+  RBLAS_HIP_TRY(xf.reserve(sx, sy), "pinned staging buffer");
   hipStream_t st = ctx.stream;
-  RBLAS_HIP_TRY(hipMemcpyAsync(ctx.buf[0], x, ex * sizeof(T), hipMemcpyHostToDevice, st), "copy x");
-  RBLAS_HIP_TRY(hipMemcpyAsync(ctx.buf[1], y, ey * sizeof(T), hipMemcpyHostToDevice, st), "copy y");
-  RBLAS_HIP_TRY(launch_axpy<T>((int64_t)n, alpha, (const T *)ctx.buf[0], (int64_t)incx,
-                               (T *)ctx.buf[1], (int64_t)incy, st),
+  RBLAS_HIP_TRY(xf.in(sx, x, ctx.buf[0]), "copy x");
+  RBLAS_HIP_TRY(xf.in(sy, y, ctx.buf[1]), "copy y");
+  RBLAS_HIP_TRY(launch_axpy<T>((int64_t)n, alpha, (const T *)ctx.buf[0], (int64_t)sx.dev_ld,
+                               (T *)ctx.buf[1], (int64_t)sy.dev_ld, st),
                 "axpy launch");
-  RBLAS_HIP_TRY(hipMemcpyAsync(y, ctx.buf[1], ey * sizeof(T), hipMemcpyDeviceToHost, st), "copy y");
-  RBLAS_HIP_TRY(hipStreamSynchronize(st), "axpy");
+  RBLAS_HIP_TRY(xf.out(sy, ctx.buf[1], y), "copy y");
+  RBLAS_HIP_TRY(xf.finish(), "axpy");
   return RBLAS_OK;
 }
 

@@ -2,6 +2,7 @@
 // entry points of the C ABI.
 #include "runtime.hpp"
 
+#include <cstring>
 #include <mutex>
 #include <vector>
 
@@ -65,6 +66,91 @@ hipError_t Ctx::ensure(int slot, size_t bytes) {
     return e;
   }
   cap[slot] = want;
+  return hipSuccess;
+}
+
+hipError_t Ctx::ensure_pinned(size_t bytes) {
+  if (bytes <= pinned_cap) return hipSuccess;
+  if (pinned) {
+    hipError_t e = hipStreamSynchronize(stream);
+    if (e != hipSuccess) return e;
+    hipHostFree(pinned);
+    pinned = nullptr;
+    pinned_cap = 0;
+  }
+  size_t want = bytes < (size_t(1) << 20) ? (size_t(1) << 20) : bytes;
+  hipError_t e = hipHostMalloc((void **)&pinned, want, hipHostMallocDefault);
+  if (e != hipSuccess) {
+    pinned = nullptr;
+    return e;
+  }
+  pinned_cap = want;
+  return hipSuccess;
+}
+
+namespace {
+
+// Copies a rows x cols column-major operand (ld_src -> ld_dst, in elements
+// of `elem` bytes) on the host.
+template <typename E>
+void copy_strided(E *dst, size_t ld_dst, const E *src, size_t ld_src, size_t cols) {
+  for (size_t c = 0; c < cols; ++c) dst[c * ld_dst] = src[c * ld_src];
+}
+
+void copy_columns(char *dst, size_t ld_dst, const char *src, size_t ld_src, size_t rows,
+                  size_t cols, size_t elem) {
+  if (ld_dst == rows && ld_src == rows) {
+    std::memcpy(dst, src, rows * cols * elem);
+    return;
+  }
+  if (rows == 1 && elem == 4) {  // strided vectors: one element per column
+    copy_strided((uint32_t *)dst, ld_dst, (const uint32_t *)src, ld_src, cols);
+    return;
+  }
+  if (rows == 1 && elem == 8) {
+    copy_strided((uint64_t *)dst, ld_dst, (const uint64_t *)src, ld_src, cols);
+    return;
+  }
+  for (size_t c = 0; c < cols; ++c)
+    std::memcpy(dst + c * ld_dst * elem, src + c * ld_src * elem, rows * elem);
+}
+
+}  // namespace
+
+hipError_t HostXfer::in(const Staged &s, const void *host, void *dev) {
+  if (s.elems == 0) return hipSuccess;
+  const size_t bytes = s.elems * elem_;
+  if (!s.pinned)  // contiguous (ld == rows) or the linear extent
+    return hipMemcpyAsync(dev, host, bytes, hipMemcpyHostToDevice, ctx_.stream);
+  char *pin = ctx_.pinned + s.pin_off;
+  if (s.packed)
+    copy_columns(pin, s.rows, (const char *)host, s.ld, s.rows, s.cols, elem_);
+  else
+    std::memcpy(pin, host, bytes);
+  return hipMemcpyAsync(dev, pin, bytes, hipMemcpyHostToDevice, ctx_.stream);
+}
+
+hipError_t HostXfer::out(const Staged &s, const void *dev, void *host) {
+  if (s.elems == 0) return hipSuccess;
+  const size_t bytes = s.elems * elem_;
+  if (!s.pinned) return hipMemcpyAsync(host, dev, bytes, hipMemcpyDeviceToHost, ctx_.stream);
+  if (nouts_ == 2) return hipErrorInvalidValue;
+  outs_[nouts_++] = Pending{&s, host};
+  return hipMemcpyAsync(ctx_.pinned + s.pin_off, dev, bytes, hipMemcpyDeviceToHost, ctx_.stream);
+}
+
+hipError_t HostXfer::finish() {
+  hipError_t e = hipStreamSynchronize(ctx_.stream);
+  if (e != hipSuccess) return e;
+  for (int i = 0; i < nouts_; ++i) {
+    const Staged &s = *outs_[i].s;
+    const char *pin = ctx_.pinned + s.pin_off;
+    if (s.packed)
+      copy_columns((char *)outs_[i].host, s.ld, pin, s.rows, s.rows, s.cols, elem_);
+    else
+      std::memcpy(outs_[i].host, pin, s.elems * elem_);
+  }
+  nouts_ = 0;
   return hipSuccess;
 }
 

@@ -62,20 +62,22 @@ int32_t syrk_host(int32_t uplo, int32_t trans, size_t n, size_t k, T alpha, cons
   Ctx &ctx = *lease.get();
   const bool t = is_transposed(trans);
   const bool need_a = k != 0 && alpha != T(0);
+  HostXfer x(ctx, sizeof(T), ((need_a ? n * k : 0) + n * n) * sizeof(T));
   Staged sa, sc;
-  if (need_a) sa.plan(t ? k : n, t ? n : k, lda);
-  sc.plan(n, n, ldc);
+  if (need_a) sa.plan(t ? k : n, t ? n : k, lda, x.pinned());
+  sc.plan(n, n, ldc, x.pinned());
   RBLAS_HIP_TRY(ctx.ensure(0, sa.elems * sizeof(T)), "device buffer A");
   RBLAS_HIP_TRY(ctx.ensure(2, sc.elems * sizeof(T)), "device buffer C");
+  RBLAS_HIP_TRY(x.reserve(sa, sc), "pinned staging buffer");
   T *da = (T *)ctx.buf[0], *dc = (T *)ctx.buf[2];
   hipStream_t st = ctx.stream;
-  if (need_a) RBLAS_HIP_TRY(stage_in(sa, a, da, st), "copy A to device");
-  RBLAS_HIP_TRY(stage_in(sc, (const T *)c, dc, st), "copy C to device");
+  if (need_a) RBLAS_HIP_TRY(x.in(sa, a, da), "copy A to device");
+  RBLAS_HIP_TRY(x.in(sc, c, dc), "copy C to device");
   RBLAS_HIP_TRY(syrk_enqueue<T>(uplo, trans, (int64_t)n, (int64_t)k, alpha, da,
                                 (int64_t)sa.dev_ld, beta, dc, (int64_t)sc.dev_ld, st),
                 "syrk launch");
-  RBLAS_HIP_TRY(stage_out(sc, dc, c, st), "copy C to host");
-  RBLAS_HIP_TRY(hipStreamSynchronize(st), "syrk");
+  RBLAS_HIP_TRY(x.out(sc, dc, c), "copy C to host");
+  RBLAS_HIP_TRY(x.finish(), "syrk");
   return RBLAS_OK;
 }
 
@@ -114,22 +116,24 @@ int32_t trsm_host(int32_t side, int32_t uplo, int32_t trans, int32_t diag, size_
   CtxLease lease;
   if (!lease.get()) return hip_fail(lease.status(), "stream create");
   Ctx &ctx = *lease.get();
+  HostXfer x(ctx, sizeof(T), (q * q + m * n) * sizeof(T));
   Staged sa, sb;
-  sa.plan(q, q, lda);
-  sb.plan(m, n, ldb);
+  sa.plan(q, q, lda, x.pinned());
+  sb.plan(m, n, ldb, x.pinned());
   RBLAS_HIP_TRY(ctx.ensure(0, sa.elems * sizeof(T)), "device buffer A");
   RBLAS_HIP_TRY(ctx.ensure(2, sb.elems * sizeof(T)), "device buffer B");
   RBLAS_HIP_TRY(ctx.ensure(3, trsm_workspace_bytes((int64_t)m, (int64_t)n, left)),
                 "device workspace");
+  RBLAS_HIP_TRY(x.reserve(sa, sb), "pinned staging buffer");
   T *da = (T *)ctx.buf[0], *db = (T *)ctx.buf[2];
   hipStream_t st = ctx.stream;
-  RBLAS_HIP_TRY(stage_in(sa, a, da, st), "copy A to device");
-  RBLAS_HIP_TRY(stage_in(sb, (const T *)b, db, st), "copy B to device");
+  RBLAS_HIP_TRY(x.in(sa, a, da), "copy A to device");
+  RBLAS_HIP_TRY(x.in(sb, b, db), "copy B to device");
   TrsmProblem<T> p = trsm_problem<T>(side, uplo, trans, diag, m, n, alpha, da, sa.dev_ld, db,
                                      sb.dev_ld);
   RBLAS_HIP_TRY(launch_trsm<T>(p, ctx.buf[3], st), "trsm launch");
-  RBLAS_HIP_TRY(stage_out(sb, db, b, st), "copy B to host");
-  RBLAS_HIP_TRY(hipStreamSynchronize(st), "trsm");
+  RBLAS_HIP_TRY(x.out(sb, db, b), "copy B to host");
+  RBLAS_HIP_TRY(x.finish(), "trsm");
   return RBLAS_OK;
 }
 
