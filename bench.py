@@ -27,6 +27,7 @@ synthetic data of the stated shape:
   - DGEMM and SGEMM 8192^3 in all 8 Layout x Transpose variants;
   - the small SGEMM (col-major NN 256^3, U[-1,1] seed 0, beta 0.5): device
     time per call and host-slice call time;
+  - the headline DGEMM through the host-slice entry point (PCIe-inclusive);
   - C64 gemm (4M) 4096^3 and Level-1 bandwidth (f64 and f32).
 `roofline` prices the DGEMM kernel against MI355X's FP64 MFMA peak, with HBM
 traffic per launch from the committed rocprofv3 PMC summary; `cpu_baseline`
@@ -231,6 +232,22 @@ def sgemm_256(calls=200):
     return {"device_us_per_call": round(ms * 1e3, 2),
             "device_tflops": round(2.0 * n ** 3 / ms / 1e9, 3),
             "host_call_us": round(host_us, 1)}
+
+
+def dgemm_host_call(n=8192, calls=2):
+    """The headline DGEMM through the host-slice entry point (operands in
+    host memory: copies in over PCIe, the kernel, C copied back,
+    synchronous): ms per call and the PCIe-inclusive TFLOP/s."""
+    from rblas_amd import level3, workloads
+    ld = n + 7
+    la = (n - 1) * ld + n
+    a, b, c = (workloads.normal(1, i * la, la, np.float64) for i in range(3))
+    level3.gemm(1, 0, 0, n, n, n, 1.5, a, ld, b, ld, -0.25, c, ld)
+    t0 = time.perf_counter()
+    for _ in range(calls):
+        level3.gemm(1, 0, 0, n, n, n, 1.5, a, ld, b, ld, -0.25, c, ld)
+    ms = (time.perf_counter() - t0) / calls * 1e3
+    return {"ms_per_call": round(ms, 2), "tflops_pcie_inclusive": round(2.0 * n ** 3 / ms / 1e9, 2)}
 
 
 def skinny_sgemm_tflops(steps=5):
@@ -466,6 +483,7 @@ ONLY = {
     "dgemm_variants": lambda a: gemm_variants(np.float64),
     "sgemm_variants": lambda a: gemm_variants(np.float32),
     "sgemm_256": lambda a: sgemm_256(),
+    "dgemm_host": lambda a: dgemm_host_call(),
 }
 
 
@@ -533,6 +551,7 @@ def main():
         comp["dgemm_8192_variants_tflops"] = gemm_variants(np.float64)
         comp["sgemm_8192_variants_tflops"] = gemm_variants(np.float32)
         comp["sgemm_256"] = sgemm_256()
+        comp["dgemm_8192_host_call"] = dgemm_host_call()
         log("grouped gemm ...")
         comp["grouped_gemm_f64"] = grouped_gemm(np.float64)
         comp["grouped_gemm_f32"] = grouped_gemm(np.float32)
