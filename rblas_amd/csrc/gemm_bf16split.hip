@@ -85,20 +85,23 @@ __global__ __launch_bounds__(256) void split_kernel(const float *__restrict__ x,
   __shared__ float tile[64][65];
   const int k0 = blockIdx.x * 64, r0 = blockIdx.y * 64;
   const int t = threadIdx.x;
+  // All 16 loads issued before any is used, without branches (an element
+  // outside the operand reads x[0] and is replaced by 0).
+  float raw[16];
 #pragma unroll
   for (int s = 0; s < 16; ++s) {
-    int rl, kl;
-    if (R_CONTIG) {
-      rl = t & 63;
-      kl = (t >> 6) + 4 * s;
-    } else {
-      kl = t & 63;
-      rl = (t >> 6) + 4 * s;
-    }
+    const int rl = R_CONTIG ? (t & 63) : (t >> 6) + 4 * s;
+    const int kl = R_CONTIG ? (t >> 6) + 4 * s : (t & 63);
     const int r = r0 + rl, k = k0 + kl;
-    float v = 0.f;
-    if (r < rows && k < K) v = R_CONTIG ? x[r + (int64_t)k * ld] : x[k + (int64_t)r * ld];
-    tile[rl][kl] = v;
+    const bool ok = r < rows && k < K;
+    const float *q = R_CONTIG ? x + r + (int64_t)k * ld : x + k + (int64_t)r * ld;
+    raw[s] = *(ok ? q : x);
+  }
+#pragma unroll
+  for (int s = 0; s < 16; ++s) {
+    const int rl = R_CONTIG ? (t & 63) : (t >> 6) + 4 * s;
+    const int kl = R_CONTIG ? (t >> 6) + 4 * s : (t & 63);
+    tile[rl][kl] = (r0 + rl < rows && k0 + kl < K) ? raw[s] : 0.f;
   }
   __syncthreads();
   // Thread t writes row r0 + t%64, k-block k0/16 + t/64 (16 values, 32 B per plane).
