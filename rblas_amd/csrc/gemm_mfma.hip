@@ -110,8 +110,9 @@ hipError_t launch_tiles(const GemmProblem<T> &p, int splits, T *ws, int *counter
 
 template <typename T>
 bool gemm_fits(const GemmProblem<T> &p) {
-  // Largest in-tile byte offset for either operand orientation.
-  const int64_t ld = p.lda > p.ldb ? p.lda : p.ldb;
+  // Largest in-tile byte offset for either operand orientation or for C.
+  int64_t ld = p.lda > p.ldb ? p.lda : p.ldb;
+  if (p.ldc > ld) ld = p.ldc;
   const int64_t span = (int64_t)(kMaxTile - 1) * ld + kMaxTile + Geo<T>::BK;
   return span * (int64_t)sizeof(T) < 0x7fffffffll - 16;
 }

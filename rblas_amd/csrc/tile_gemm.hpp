@@ -132,6 +132,20 @@ __device__ __forceinline__ float buffer_load<float>(__amdgpu_buffer_rsrc_t r, ui
   return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
 }
 
+template <typename T>
+__device__ __forceinline__ void buffer_store(__amdgpu_buffer_rsrc_t r, uint32_t off, T v);
+template <>
+__device__ __forceinline__ void buffer_store<double>(__amdgpu_buffer_rsrc_t r, uint32_t off,
+                                                     double v) {
+  typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
+  __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, off, 0, 0);
+}
+template <>
+__device__ __forceinline__ void buffer_store<float>(__amdgpu_buffer_rsrc_t r, uint32_t off,
+                                                    float v) {
+  __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), r, off, 0, 0);
+}
+
 // Byte offset of tile element (k, mn) inside one operand's LDS tile.
 template <typename T, bool MN_MAJOR, int TILE>
 __device__ __forceinline__ int lds_byte(int k, int mn) {
@@ -434,11 +448,14 @@ struct TileGemm {
   }
 
   // C := alpha * acc + beta * C on the tile at (m0, n0); beta == 0 never
-  // reads C; tri 1 / 2 keeps only the upper / lower triangle. The C loads of
-  // kEpiRows block rows are issued together, without branches (an element
-  // outside the matrix or the triangle reads C[0] instead and is not
-  // stored), so a tile waits on memory FM / kEpiRows times, not once per
-  // element.
+  // reads C; tri 1 / 2 keeps only the upper / lower triangle. C is accessed
+  // through a buffer resource based at the tile's origin: each lane has one
+  // byte offset, each element adds a wave-uniform offset, and an element
+  // outside the matrix or the triangle uses an offset past num_records, so
+  // its load returns 0 and its store is dropped -- no branches and no
+  // per-element 64-bit address arithmetic (gemm_fits keeps every in-tile
+  // offset of C below 2^31). The C loads of kEpiRows block rows are issued
+  // together before their stores.
   static constexpr int kEpiRows = FM * FN * 4 <= 32 ? FM : FM / 2;
 
   __device__ __forceinline__ bool c_in(const GemmProblem<T> &p, int gi, int gj) const {
@@ -449,37 +466,44 @@ struct TileGemm {
   __device__ __forceinline__ void epilogue(const GemmProblem<T> &p, int m0, int n0) const {
     const T alpha = p.alpha, beta = p.beta;
     const bool read_c = beta != T(0);  // uniform
+    const int64_t ldc = p.ldc;
+    const T *c0 = p.c + m0 + (int64_t)n0 * ldc;
+    const uint64_t span = (uint64_t)((p.n - 1 - n0) * ldc + (p.m - m0)) * sizeof(T);
+    const __amdgpu_buffer_rsrc_t rs = make_rsrc(c0, span);
+    // Lane part of the offset: row row_of(0), column col_of(0, 0).
+    const uint32_t lane_byte = (uint32_t)((row_of(0) + (int64_t)col_of(0, 0) * ldc) * sizeof(T));
 #pragma unroll
     for (int i0 = 0; i0 < FM; i0 += kEpiRows) {
       T cv[kEpiRows][FN][4];
       if (read_c) {
 #pragma unroll
-        for (int ii = 0; ii < kEpiRows; ++ii) {
-          const int gi = m0 + row_of(i0 + ii);
+        for (int ii = 0; ii < kEpiRows; ++ii)
 #pragma unroll
           for (int j = 0; j < FN; ++j)
 #pragma unroll
-            for (int r = 0; r < 4; ++r) {
-              const int gj = n0 + col_of(j, r);
-              const T *q = c_in(p, gi, gj) ? p.c + gi + (int64_t)gj * p.ldc : p.c;
-              cv[ii][j][r] = *q;
-            }
-        }
+            for (int r = 0; r < 4; ++r)
+              cv[ii][j][r] = buffer_load<T>(rs, elem_off(p, m0, n0, i0 + ii, j, r, lane_byte));
       }
 #pragma unroll
-      for (int ii = 0; ii < kEpiRows; ++ii) {
-        const int gi = m0 + row_of(i0 + ii);
+      for (int ii = 0; ii < kEpiRows; ++ii)
 #pragma unroll
         for (int j = 0; j < FN; ++j)
 #pragma unroll
           for (int r = 0; r < 4; ++r) {
-            const int gj = n0 + col_of(j, r);
             const T v = acc[i0 + ii][j][r];
             const T out = read_c ? __builtin_fma(alpha, v, beta * cv[ii][j][r]) : alpha * v;
-            if (c_in(p, gi, gj)) p.c[gi + (int64_t)gj * p.ldc] = out;
+            buffer_store<T>(rs, elem_off(p, m0, n0, i0 + ii, j, r, lane_byte), out);
           }
-      }
     }
+  }
+
+  // Byte offset of this lane's element (i, j, r) from the tile origin, or
+  // kMasked when the element is outside C or the kept triangle.
+  __device__ __forceinline__ uint32_t elem_off(const GemmProblem<T> &p, int m0, int n0, int i,
+                                               int j, int r, uint32_t lane_byte) const {
+    const int di = row_of(i) - row_of(0), dj = col_of(j, r) - col_of(0, 0);  // uniform
+    const uint32_t u = (uint32_t)((di + (int64_t)dj * p.ldc) * sizeof(T));
+    return c_in(p, m0 + row_of(i), n0 + col_of(j, r)) ? lane_byte + u : kMasked;
   }
 
   static constexpr int kAccPerThread = FM * FN * 4;
