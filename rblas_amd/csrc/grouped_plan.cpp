@@ -11,11 +11,11 @@ namespace rblas {
 namespace {
 
 // Defaults: the autotuner's measurement on MI355X (rblas_amd/autotune.py,
-// saved in rblas_amd/tuning_gfx950.json). The unit cap sits just above where
-// splitting stopped paying on the batching workload, so only very long-k
-// units are split.
-Tuning g_tuning = {{{6.582, 16.869}, {6.517, 17.415}},
-                   {{0.791, 2.306}, {0.737, 2.136}},
+// saved in rblas_amd/tuning_gfx950.json). The unit cap sits where splitting
+// stopped paying on the batching workload (for f64 a cap of 320 us and no
+// cap at all timed the same; 320 keeps very long-k units splittable).
+Tuning g_tuning = {{{4.746, 9.366}, {4.291, 10.249}},
+                   {{0.793, 2.305}, {0.734, 2.153}},
                    {320.0, 320.0}};
 std::mutex g_mu;
 
