@@ -497,7 +497,7 @@ def main():
         return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
-                          "(multi-GPU sharding is described in DESIGN.md, not built)"}))
+                          "(multi-GPU sharding is designed but not built)"}))
         sys.exit(2)
     from rblas_amd import workloads
     from rblas_amd.types import Compute
