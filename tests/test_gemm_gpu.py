@@ -125,6 +125,48 @@ def test_reference_fixture_17x13x19(gpu):
     assert np.abs(c - want).max() < 1e-9
 
 
+def test_reference_fixture_8cubed_alpha_beta(gpu):
+    # kernel/gemm.rs:704-717 through the GPU: f32, alpha 2, beta 3, C = 1.
+    # Every product and partial sum is an integer below 2^24, so the whole C
+    # is exact; rblas checks C[0] to 1e-3.
+    m = n = k = 8
+    a = np.arange(m * k, dtype=np.float32)
+    b = (np.arange(k * n) % 5).astype(np.float32)
+    c = np.ones(m * n, np.float32)
+    level3.gemm(COL, 0, 0, m, n, k, 2.0, a, m, b, k, 3.0, c, m)
+    acc = sum(float(a[p * m]) * float(b[p]) for p in range(k))
+    assert abs(c[0] - (2 * acc + 3)) < 1e-3
+    A = a.reshape(k, m).T.astype(np.float64)
+    B = b.reshape(n, k).T.astype(np.float64)
+    assert np.array_equal(c.reshape(n, m).T.astype(np.float64), 2 * A @ B + 3)
+
+
+@pytest.mark.parametrize("case", ["nn_colmajor", "tn_colmajor", "rowmajor_nn"])
+def test_reference_layout_kats(gpu, case):
+    # level3.rs:642-736 through the GPU, against the naive product, 1e-9.
+    if case == "nn_colmajor":
+        m, n, k = 6, 5, 7
+        a = np.arange(m * k, dtype=np.float64)
+        b = np.arange(k * n, dtype=np.float64) * 2
+        args = (COL, 0, 0, m, n, k, 1.0, a, m, b, k)
+    elif case == "tn_colmajor":
+        m, n, k = 4, 5, 6
+        a = np.sqrt(np.arange(k * m, dtype=np.float64))
+        b = np.arange(k * n, dtype=np.float64) * 0.1
+        args = (COL, 1, 0, m, n, k, 1.0, a, k, b, k)
+    else:
+        m, n, k = 5, 4, 3
+        a = np.arange(m * k, dtype=np.float64)
+        b = np.arange(k * n, dtype=np.float64) + 1
+        args = (ROW, 0, 0, m, n, k, 1.0, a, k, b, n)
+    layout = args[0]
+    ldc = m if layout == COL else n
+    c = np.zeros(m * n)
+    level3.gemm(*args, 0.0, c, ldc)
+    want, _ = gemm_expect(*args, 0.0, np.zeros(m * n), ldc)
+    np.testing.assert_allclose(view(c, layout, m, n, ldc), want, rtol=0, atol=1e-9)
+
+
 def test_reference_fixture_lcg_200x180x150(gpu):
     # tests/correctness.rs:29-58 through the GPU, max rel < 1e-3 as in rblas
     m, n, k = 200, 180, 150
