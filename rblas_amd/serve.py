@@ -59,7 +59,7 @@ def workload(count: int = 10_000, rate: float = 0.0, seed: int = 7) -> np.ndarra
     return req
 
 
-def run(requests: np.ndarray, mode: int = BATCHED, max_inflight: int = 3, max_batch: int = 64,
+def run(requests: np.ndarray, mode: int = BATCHED, max_inflight: int = 4, max_batch: int = 32,
         hbm_cap: float = HBM_CAP, capture: dict | None = None):
     """Serve `requests` (REQUEST_DTYPE). `capture` maps request index -> a
     host array that receives its dense result. Returns (results, stats)."""
