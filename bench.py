@@ -18,6 +18,8 @@ synthetic data of the stated shape:
     same problems as one launch each;
   - single trsm calls (Left / Lower / NoTrans, q = nrhs in {1024, 2048,
     8192}, f64 and f32);
+  - DSYMM 8192^3 and ZHEMM 4096^3 (dense A from the stored triangle, then
+    the gemm);
   - grouped trsm (256 lower systems, n log-uniform [32, 1024], nrhs in
     [1, 256], diag 2+|N(0,1)|, off-diagonal N(0,1)/sqrt(n), f64) against
     per-call trsm;
@@ -357,6 +359,35 @@ def grouped_trsm():
             "roofline_frac": round(ideal_s * 1e3 / ms, 3)}
 
 
+def symm_hemm():
+    """DSYMM (Left, Upper, m = n = 8192, ld 8199; 2 m^2 n flops) and ZHEMM
+    (Left, Lower, m = n = 4096; 8 m^2 n real flops), resident operands."""
+    from rblas_amd import complex3, level3, workloads
+    from rblas_amd.device import DeviceArray, Stream
+    s = Stream()
+    out = {}
+    n = 8192
+    ld = n + 7
+    la = (n - 1) * ld + n
+    da, db, dc = (DeviceArray.from_host(workloads.normal(5, i * la, la, np.float64), s)
+                  for i in range(3))
+    ms = timed(lambda: level3.symm_dev(0, 0, n, n, 1.5, da, ld, db, ld, -0.25, dc, ld, s), s, 5, 1)
+    out["dsymm_8192"] = {"ms": round(ms, 3), "tflops": round(2.0 * n ** 3 / ms / 1e9, 2)}
+    for x in (da, db, dc):
+        x.free()
+    n = 4096
+    z = [workloads.normal(6, 2 * i * n * n, 2 * n * n, np.float64).view(np.complex128)
+         for i in range(3)]
+    da, db, dc = (DeviceArray.from_host(x, s) for x in z)
+    ms = timed(lambda: complex3.hemm_dev(0, 1, n, n, 1.5 - 0.5j, da, n, db, n, 0.25, dc, n, s),
+               s, 5, 1)
+    out["zhemm_4096"] = {"ms": round(ms, 3), "tflops": round(8.0 * n ** 3 / ms / 1e9, 2)}
+    for x in (da, db, dc):
+        x.free()
+    s.close()
+    return out
+
+
 def trsm_single():
     """Single trsm calls on resident operands: Left / Lower / NoTrans,
     q = nrhs in {1024, 2048, 8192}, f64 and f32 (TFLOP/s in q^2 nrhs)."""
@@ -476,6 +507,7 @@ ONLY = {
     "grouped_gemm_f32": lambda a: grouped_gemm(np.float32),
     "grouped_trsm_f64": lambda a: grouped_trsm(),
     "trsm": lambda a: trsm_single(),
+    "symm_hemm": lambda a: symm_hemm(),
     "serve": lambda a: serving(a.serve_requests),
     "c64_gemm": lambda a: c64_gemm(),
     "sgemm_skinny": lambda a: skinny_sgemm_tflops(),
@@ -557,6 +589,8 @@ def main():
         comp["grouped_gemm_f32"] = grouped_gemm(np.float32)
         log("trsm ...")
         comp["trsm"] = trsm_single()
+        log("symm / hemm ...")
+        comp["symm_hemm"] = symm_hemm()
         log("grouped trsm ...")
         comp["grouped_trsm_f64"] = grouped_trsm()
         log("serving ...")

@@ -237,6 +237,79 @@ int32_t rblas_dtrsm_dev(int32_t side, int32_t uplo, int32_t trans, int32_t diag,
                         size_t a_len, size_t lda, double *b, size_t b_len,
                         size_t ldb, void *stream);
 
+/* -------------------------------------------------------------- symm/hemm
+ * C := alpha * A * B + beta * C (side Left, A m x m) or alpha * B * A +
+ * beta * C (side Right, A n x n), B and C m x n, column-major, where A is
+ * symmetric with only its `uplo` triangle stored (ld lda). Replaces
+ * `level3::symm` (level3.rs:301-381): rblas builds the dense A from the
+ * stored triangle, then calls gemm(ColMajor, N, N). RBLAS_PANIC where rblas
+ * indexes past a slice: A's largest stored index (q-1)*(lda+1) first, then
+ * that gemm's checks on B and C (including the `(k-1)` forms of the NN
+ * path). Here the dense A is written in HBM by one launch and the product
+ * runs on the MFMA gemm. */
+int32_t rblas_ssymm(int32_t side, int32_t uplo, size_t m, size_t n, float alpha,
+                    const float *a, size_t a_len, size_t lda, const float *b,
+                    size_t b_len, size_t ldb, float beta, float *c, size_t c_len,
+                    size_t ldc);
+int32_t rblas_dsymm(int32_t side, int32_t uplo, size_t m, size_t n, double alpha,
+                    const double *a, size_t a_len, size_t lda, const double *b,
+                    size_t b_len, size_t ldb, double beta, double *c, size_t c_len,
+                    size_t ldc);
+/* HBM-resident forms (no rblas counterpart); the dense A and the gemm
+ * workspace come from the stream-ordered pool. */
+int32_t rblas_ssymm_dev(int32_t side, int32_t uplo, size_t m, size_t n, float alpha,
+                        const float *a, size_t a_len, size_t lda, const float *b,
+                        size_t b_len, size_t ldb, float beta, float *c, size_t c_len,
+                        size_t ldc, void *stream);
+int32_t rblas_dsymm_dev(int32_t side, int32_t uplo, size_t m, size_t n, double alpha,
+                        const double *a, size_t a_len, size_t lda, const double *b,
+                        size_t b_len, size_t ldb, double beta, double *c, size_t c_len,
+                        size_t ldc, void *stream);
+/* Complex symmetric (A = A^T) and Hermitian (A = A^H: the mirrored half is
+ * conjugated and the diagonal's imaginary part is taken as zero) forms.
+ * Replace `complex3::symm` / `complex3::hemm` (complex3.rs:127-247,
+ * he_or_sy): nothing at all when m or n is 0; otherwise RBLAS_PANIC on A's
+ * largest stored index, then complex3::gemm(N, N)'s checks on B and C. The
+ * product runs on the 4M complex gemm. Lengths and leading dimensions count
+ * complex elements. */
+int32_t rblas_csymm(int32_t side, int32_t uplo, size_t m, size_t n, rblas_complex32_t alpha,
+                    const rblas_complex32_t *a, size_t a_len, size_t lda,
+                    const rblas_complex32_t *b, size_t b_len, size_t ldb,
+                    rblas_complex32_t beta, rblas_complex32_t *c, size_t c_len, size_t ldc);
+int32_t rblas_zsymm(int32_t side, int32_t uplo, size_t m, size_t n, rblas_complex64_t alpha,
+                    const rblas_complex64_t *a, size_t a_len, size_t lda,
+                    const rblas_complex64_t *b, size_t b_len, size_t ldb,
+                    rblas_complex64_t beta, rblas_complex64_t *c, size_t c_len, size_t ldc);
+int32_t rblas_chemm(int32_t side, int32_t uplo, size_t m, size_t n, rblas_complex32_t alpha,
+                    const rblas_complex32_t *a, size_t a_len, size_t lda,
+                    const rblas_complex32_t *b, size_t b_len, size_t ldb,
+                    rblas_complex32_t beta, rblas_complex32_t *c, size_t c_len, size_t ldc);
+int32_t rblas_zhemm(int32_t side, int32_t uplo, size_t m, size_t n, rblas_complex64_t alpha,
+                    const rblas_complex64_t *a, size_t a_len, size_t lda,
+                    const rblas_complex64_t *b, size_t b_len, size_t ldb,
+                    rblas_complex64_t beta, rblas_complex64_t *c, size_t c_len, size_t ldc);
+/* HBM-resident forms (no rblas counterpart). */
+int32_t rblas_csymm_dev(int32_t side, int32_t uplo, size_t m, size_t n,
+                        rblas_complex32_t alpha, const rblas_complex32_t *a, size_t a_len,
+                        size_t lda, const rblas_complex32_t *b, size_t b_len, size_t ldb,
+                        rblas_complex32_t beta, rblas_complex32_t *c, size_t c_len,
+                        size_t ldc, void *stream);
+int32_t rblas_zsymm_dev(int32_t side, int32_t uplo, size_t m, size_t n,
+                        rblas_complex64_t alpha, const rblas_complex64_t *a, size_t a_len,
+                        size_t lda, const rblas_complex64_t *b, size_t b_len, size_t ldb,
+                        rblas_complex64_t beta, rblas_complex64_t *c, size_t c_len,
+                        size_t ldc, void *stream);
+int32_t rblas_chemm_dev(int32_t side, int32_t uplo, size_t m, size_t n,
+                        rblas_complex32_t alpha, const rblas_complex32_t *a, size_t a_len,
+                        size_t lda, const rblas_complex32_t *b, size_t b_len, size_t ldb,
+                        rblas_complex32_t beta, rblas_complex32_t *c, size_t c_len,
+                        size_t ldc, void *stream);
+int32_t rblas_zhemm_dev(int32_t side, int32_t uplo, size_t m, size_t n,
+                        rblas_complex64_t alpha, const rblas_complex64_t *a, size_t a_len,
+                        size_t lda, const rblas_complex64_t *b, size_t b_len, size_t ldb,
+                        rblas_complex64_t beta, rblas_complex64_t *c, size_t c_len,
+                        size_t ldc, void *stream);
+
 /* ----------------------------------------------------------------- level-1
  * Replace `level1::dot` (level1.rs:45), `level1::nrm2` (level1.rs:96) and
  * `level1::axpy` (level1.rs:17). Strides are rblas's `usize` increments.

@@ -284,3 +284,95 @@ def trsm_panics(side, uplo, trans, diag, m, n, alpha, a_len, lda, b_len, ldb):
                     ra(kk, j)
                 ra(j, j)
     return bad[0]
+
+
+# ------------------------------------------------------------ symm / hemm
+def full_symmetric(uplo, q, a, lda, herm=False):
+    """level3.rs:358-381 (real) / complex3.rs:188-213 (he_or_sy), walked
+    literally: the dense q x q column-major matrix from the stored `uplo`
+    triangle; herm conjugates the mirrored half and zeroes the diagonal's
+    imaginary part."""
+    full = np.zeros(q * q, a.dtype)
+    for j in range(q):
+        for i in range(q):
+            here = i <= j if uplo == 0 else i >= j
+            if here:
+                v = a[i + j * lda]
+                if herm and i == j:
+                    v = v.real + 0j
+            else:
+                v = a[j + i * lda]
+                if herm:
+                    v = np.conj(v)
+            full[i + j * q] = v
+    return full
+
+
+def symm(side, uplo, m, n, alpha, a, lda, b, ldb, beta, c, ldc):
+    """level3::symm (level3.rs:301-356): full_symmetric, then gemm(ColMajor,
+    N, N) with A left (side 0) or right (side 1). In place on c."""
+    q = m if side == 0 else n
+    full = full_symmetric(uplo, q, a, lda)
+    if side == 0:
+        gemm(COL_MAJOR, NO_TRANS, NO_TRANS, m, n, m, alpha, full, q, b, ldb, beta, c, ldc)
+    else:
+        gemm(COL_MAJOR, NO_TRANS, NO_TRANS, m, n, n, alpha, b, ldb, full, q, beta, c, ldc)
+
+
+def hesy(herm, side, uplo, m, n, alpha, a, lda, b, ldb, beta, c, ldc):
+    """complex3::symm / hemm (complex3.rs:127-247, he_or_sy): nothing when m
+    or n is 0; full_symmetric (Hermitian when herm), then complex3::gemm(N,
+    N). In place on c."""
+    if m == 0 or n == 0:
+        return
+    q = m if side == 0 else n
+    full = full_symmetric(uplo, q, a, lda, herm)
+    if side == 0:
+        cgemm(NO_TRANS, NO_TRANS, m, n, m, alpha, full, q, b, ldb, beta, c, ldc)
+    else:
+        cgemm(NO_TRANS, NO_TRANS, m, n, n, alpha, b, ldb, full, q, beta, c, ldc)
+
+
+def symm_panics(side, uplo, m, n, alpha, a_len, lda, b_len, ldb, beta, c_len, ldc):
+    """level3::symm: full_symmetric indexes a over the stored triangle (all of
+    its indices, walked), then the gemm on the dense q x q copy."""
+    q = m if side == 0 else n
+    for j in range(q):
+        for i in range(q):
+            here = i <= j if uplo == 0 else i >= j
+            if (i + j * lda if here else j + i * lda) >= a_len:
+                return True
+    if side == 0:
+        return gemm_panics(COL_MAJOR, NO_TRANS, NO_TRANS, m, n, m, q * q, q, b_len, ldb, c_len,
+                           ldc, alpha, beta)
+    return gemm_panics(COL_MAJOR, NO_TRANS, NO_TRANS, m, n, n, b_len, ldb, q * q, q, c_len, ldc,
+                       alpha, beta)
+
+
+def hesy_panics(side, uplo, m, n, a_len, lda, b_len, ldb, c_len, ldc):
+    """complex3::symm / hemm: nothing when m or n is 0; full_symmetric's
+    indices of a, then complex3::gemm(N, N)'s: split_op gathers op(A) and
+    op(B) (complex3.rs:111-118) and the combine updates C (:61-73)."""
+    if m == 0 or n == 0:
+        return False
+    q = m if side == 0 else n
+    for j in range(q):
+        for i in range(q):
+            here = i <= j if uplo == 0 else i >= j
+            if (i + j * lda if here else j + i * lda) >= a_len:
+                return True
+    k = q
+    la, lda2, lb, ldb2 = (q * q, q, b_len, ldb) if side == 0 else (b_len, ldb, q * q, q)
+    for p in range(k):          # op(A)(i, p) of the m x k left operand
+        for i in range(m):
+            if i + p * lda2 >= la:
+                return True
+    for j in range(n):          # op(B)(p, j) of the k x n right operand
+        for p in range(k):
+            if p + j * ldb2 >= lb:
+                return True
+    for j in range(n):
+        for i in range(m):
+            if i + j * ldc >= c_len:
+                return True
+    return False

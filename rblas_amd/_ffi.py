@@ -116,6 +116,26 @@ SIGNATURES: dict[str, tuple[list, object]] = {
     "rblas_dtrsm": ([i32, i32, i32, i32, sz, sz, f64, vp, sz, sz, vp, sz, sz], i32),
     "rblas_strsm_dev": ([i32, i32, i32, i32, sz, sz, f32, vp, sz, sz, vp, sz, sz, vp], i32),
     "rblas_dtrsm_dev": ([i32, i32, i32, i32, sz, sz, f64, vp, sz, sz, vp, sz, sz, vp], i32),
+    "rblas_ssymm": ([i32, i32, sz, sz, f32, vp, sz, sz, vp, sz, sz, f32, vp, sz, sz], i32),
+    "rblas_dsymm": ([i32, i32, sz, sz, f64, vp, sz, sz, vp, sz, sz, f64, vp, sz, sz], i32),
+    "rblas_ssymm_dev": ([i32, i32, sz, sz, f32, vp, sz, sz, vp, sz, sz, f32, vp, sz, sz, vp], i32),
+    "rblas_dsymm_dev": ([i32, i32, sz, sz, f64, vp, sz, sz, vp, sz, sz, f64, vp, sz, sz, vp], i32),
+    "rblas_csymm": ([i32, i32, sz, sz, Complex32, vp, sz, sz, vp, sz, sz, Complex32, vp, sz, sz],
+                    i32),
+    "rblas_zsymm": ([i32, i32, sz, sz, Complex64, vp, sz, sz, vp, sz, sz, Complex64, vp, sz, sz],
+                    i32),
+    "rblas_chemm": ([i32, i32, sz, sz, Complex32, vp, sz, sz, vp, sz, sz, Complex32, vp, sz, sz],
+                    i32),
+    "rblas_zhemm": ([i32, i32, sz, sz, Complex64, vp, sz, sz, vp, sz, sz, Complex64, vp, sz, sz],
+                    i32),
+    "rblas_csymm_dev": ([i32, i32, sz, sz, Complex32, vp, sz, sz, vp, sz, sz, Complex32, vp, sz,
+                         sz, vp], i32),
+    "rblas_zsymm_dev": ([i32, i32, sz, sz, Complex64, vp, sz, sz, vp, sz, sz, Complex64, vp, sz,
+                         sz, vp], i32),
+    "rblas_chemm_dev": ([i32, i32, sz, sz, Complex32, vp, sz, sz, vp, sz, sz, Complex32, vp, sz,
+                         sz, vp], i32),
+    "rblas_zhemm_dev": ([i32, i32, sz, sz, Complex64, vp, sz, sz, vp, sz, sz, Complex64, vp, sz,
+                         sz, vp], i32),
     "rblas_sdot": ([sz, vp, sz, sz, vp, sz, sz, C.POINTER(f32)], i32),
     "rblas_ddot": ([sz, vp, sz, sz, vp, sz, sz, C.POINTER(f64)], i32),
     "rblas_snrm2": ([sz, vp, sz, sz, C.POINTER(f32)], i32),

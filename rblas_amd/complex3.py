@@ -45,3 +45,44 @@ def gemm_dev(transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, strea
         b.size, ldb, cs(be.real, be.imag), c.ptr, c.size, ldc,
         stream.handle if stream is not None else None)
     raise_status(st)
+
+
+def _hesy(name, side, uplo, m, n, alpha, a, lda, b, ldb, beta, c, ldc) -> None:
+    pre, cs = _kind(a, b, c)
+    al, be = complex(alpha), complex(beta)
+    st = getattr(_ffi.lib(), f"rblas_{pre}{name}")(
+        int(side), int(uplo), m, n, cs(al.real, al.imag), a.ctypes.data, a.size, lda,
+        b.ctypes.data, b.size, ldb, cs(be.real, be.imag), c.ctypes.data, c.size, ldc)
+    raise_status(st)
+
+
+def _hesy_dev(name, side, uplo, m, n, alpha, a, lda, b, ldb, beta, c, ldc, stream) -> None:
+    pre = {np.dtype(np.complex64): "c", np.dtype(np.complex128): "z"}[c.dtype]
+    cs = _ffi.Complex32 if pre == "c" else _ffi.Complex64
+    al, be = complex(alpha), complex(beta)
+    st = getattr(_ffi.lib(), f"rblas_{pre}{name}_dev")(
+        int(side), int(uplo), m, n, cs(al.real, al.imag), a.ptr, a.size, lda, b.ptr, b.size, ldb,
+        cs(be.real, be.imag), c.ptr, c.size, ldc, stream.handle if stream is not None else None)
+    raise_status(st)
+
+
+def symm(side, uplo, m, n, alpha, a, lda, b, ldb, beta, c, ldc) -> None:
+    """Complex symmetric multiply, A = A^T with its `uplo` triangle stored
+    (complex3.rs:127-146)."""
+    _hesy("symm", side, uplo, m, n, alpha, a, lda, b, ldb, beta, c, ldc)
+
+
+def hemm(side, uplo, m, n, alpha, a, lda, b, ldb, beta, c, ldc) -> None:
+    """Complex Hermitian multiply, A = A^H with its `uplo` triangle stored;
+    the diagonal's imaginary part is taken as zero (complex3.rs:148-168)."""
+    _hesy("hemm", side, uplo, m, n, alpha, a, lda, b, ldb, beta, c, ldc)
+
+
+def symm_dev(side, uplo, m, n, alpha, a, lda, b, ldb, beta, c, ldc, stream=None) -> None:
+    """symm on HBM-resident complex operands (device.DeviceArray)."""
+    _hesy_dev("symm", side, uplo, m, n, alpha, a, lda, b, ldb, beta, c, ldc, stream)
+
+
+def hemm_dev(side, uplo, m, n, alpha, a, lda, b, ldb, beta, c, ldc, stream=None) -> None:
+    """hemm on HBM-resident complex operands (device.DeviceArray)."""
+    _hesy_dev("hemm", side, uplo, m, n, alpha, a, lda, b, ldb, beta, c, ldc, stream)

@@ -117,6 +117,110 @@ int32_t cgemm_dev(int32_t ta, int32_t tb, size_t m, size_t n, size_t k, T ar, T 
   return RBLAS_OK;
 }
 
+// --------------------------------------------------------- symm / hemm
+// complex3::symm / hemm (complex3.rs:127-247, he_or_sy): nothing when m or n
+// is 0; otherwise the dense A is built from its stored `uplo` triangle
+// (Hermitian: mirrored half conjugated, diagonal imaginary part zero; every
+// read a stored entry, the largest index (q-1)(lda+1)) and the product runs
+// through complex3::gemm(N, N) with A on the left (Side Left) or right.
+template <typename T>
+Check hesy_validate(int32_t side, int32_t uplo, size_t m, size_t n, size_t a_len, size_t lda,
+                    size_t b_len, size_t ldb, size_t c_len, size_t ldc) {
+  if ((side != RBLAS_LEFT && side != RBLAS_RIGHT) || (uplo != RBLAS_UPPER && uplo != RBLAS_LOWER)) {
+    Check bad;
+    bad.status = RBLAS_BAD_ENUM;
+    bad.err = {RBLAS_BAD_ENUM, 0, 0, 0};
+    bad.msg = "symm/hemm: enum argument out of range";
+    return bad;
+  }
+  if (m == 0 || n == 0) return Check{};
+  const size_t q = side == RBLAS_LEFT ? m : n;
+  size_t hi;
+  const bool fits = max_index(q, q, 1, lda, hi);
+  if (!fits || hi >= a_len)
+    return panic("index out of bounds: the len is " + std::to_string(a_len) +
+                 " but the index is " + (fits ? std::to_string(hi) : std::string("past usize")));
+  if (side == RBLAS_LEFT)
+    return cgemm_validate<T>(0, 0, m, n, q, q * q, q, b_len, ldb, c_len, ldc);
+  return cgemm_validate<T>(0, 0, m, n, q, b_len, ldb, q * q, q, c_len, ldc);
+}
+
+// Dense-A elements of T (interleaved complex, 256-byte aligned).
+inline size_t hesy_full_elems(size_t q) { return (2 * q * q + 31) / 32 * 32; }
+
+template <typename T>
+hipError_t hesy_enqueue(bool left, bool upper, bool herm, size_t m, size_t n, T ar, T ai,
+                        const T *a, size_t lda, const T *b, size_t ldb, T br, T bi, T *c,
+                        size_t ldc, T *ws, hipStream_t st) {
+  const size_t q = left ? m : n;
+  T *full = ws;
+  hipError_t e = launch_sym_full<T>(a, (int64_t)lda, (int64_t)q, upper, true, herm, full, st);
+  if (e != hipSuccess) return e;
+  const T *pa = left ? full : b, *pb = left ? b : full;
+  const size_t la = left ? q : ldb, lb = left ? ldb : q;
+  CgemmProblem<T> p{0, 0, (int64_t)m, (int64_t)n, (int64_t)q, ar, ai, br, bi, pa, (int64_t)la,
+                    pb, (int64_t)lb, c, (int64_t)ldc};
+  return launch_cgemm<T>(p, ws + hesy_full_elems(q), st);
+}
+
+template <typename T>
+int32_t hesy_host(bool herm, int32_t side, int32_t uplo, size_t m, size_t n, T ar, T ai,
+                  const T *a, size_t a_len, size_t lda, const T *b, size_t b_len, size_t ldb, T br,
+                  T bi, T *c, size_t c_len, size_t ldc) {
+  Check chk = hesy_validate<T>(side, uplo, m, n, a_len, lda, b_len, ldb, c_len, ldc);
+  if (!chk.ok()) return fail(chk, nullptr);
+  if (m == 0 || n == 0) return RBLAS_OK;
+  const bool left = side == RBLAS_LEFT;
+  const size_t q = left ? m : n;
+  CtxLease lease;
+  if (!lease.get()) return hip_fail(lease.status(), "stream create");
+  Ctx &ctx = *lease.get();
+  const size_t cs = 2 * sizeof(T);
+  HostXfer x(ctx, cs, (q * q + 2 * m * n) * cs);
+  Staged sa, sb, sc;
+  sa.plan(q, q, lda, x.pinned());
+  sb.plan(m, n, ldb, x.pinned());
+  sc.plan(m, n, ldc, x.pinned());
+  RBLAS_HIP_TRY(ctx.ensure(0, sa.elems * cs), "device buffer A");
+  RBLAS_HIP_TRY(ctx.ensure(1, sb.elems * cs), "device buffer B");
+  RBLAS_HIP_TRY(ctx.ensure(2, sc.elems * cs), "device buffer C");
+  RBLAS_HIP_TRY(ctx.ensure(3, (hesy_full_elems(q) + cgemm_workspace_elems(m, n, q)) * sizeof(T)),
+                "device workspace");
+  RBLAS_HIP_TRY(x.reserve(sa, sb, sc), "pinned staging buffer");
+  RBLAS_HIP_TRY(x.in(sa, a, ctx.buf[0]), "copy A");
+  RBLAS_HIP_TRY(x.in(sb, b, ctx.buf[1]), "copy B");
+  RBLAS_HIP_TRY(x.in(sc, c, ctx.buf[2]), "copy C");
+  RBLAS_HIP_TRY(hesy_enqueue<T>(left, uplo == RBLAS_UPPER, herm, m, n, ar, ai,
+                                (const T *)ctx.buf[0], sa.dev_ld, (const T *)ctx.buf[1],
+                                sb.dev_ld, br, bi, (T *)ctx.buf[2], sc.dev_ld, (T *)ctx.buf[3],
+                                ctx.stream),
+                "symm/hemm launch");
+  RBLAS_HIP_TRY(x.out(sc, ctx.buf[2], c), "copy C back");
+  RBLAS_HIP_TRY(x.finish(), "symm/hemm");
+  return RBLAS_OK;
+}
+
+template <typename T>
+int32_t hesy_dev(bool herm, int32_t side, int32_t uplo, size_t m, size_t n, T ar, T ai,
+                 const T *a, size_t a_len, size_t lda, const T *b, size_t b_len, size_t ldb, T br,
+                 T bi, T *c, size_t c_len, size_t ldc, void *stream) {
+  Check chk = hesy_validate<T>(side, uplo, m, n, a_len, lda, b_len, ldb, c_len, ldc);
+  if (!chk.ok()) return fail(chk, nullptr);
+  if (m == 0 || n == 0) return RBLAS_OK;
+  const bool left = side == RBLAS_LEFT;
+  const size_t q = left ? m : n;
+  hipStream_t st = (hipStream_t)stream;
+  void *ws = nullptr;
+  RBLAS_HIP_TRY(pool_alloc(&ws, (hesy_full_elems(q) + cgemm_workspace_elems(m, n, q)) * sizeof(T),
+                           st),
+                "symm/hemm workspace");
+  hipError_t e = hesy_enqueue<T>(left, uplo == RBLAS_UPPER, herm, m, n, ar, ai, a, lda, b, ldb,
+                                 br, bi, c, ldc, (T *)ws, st);
+  hipFreeAsync(ws, st);
+  RBLAS_HIP_TRY(e, "symm/hemm launch");
+  return RBLAS_OK;
+}
+
 }  // namespace
 }  // namespace rblas
 
@@ -160,6 +264,66 @@ int32_t rblas_zgemm_dev(int32_t transa, int32_t transb, size_t m, size_t n, size
   return cgemm_dev<double>(transa, transb, m, n, k, alpha.re, alpha.im, (const double *)a, a_len,
                            lda, (const double *)b, b_len, ldb, beta.re, beta.im, (double *)c,
                            c_len, ldc, stream);
+}
+
+int32_t rblas_csymm(int32_t side, int32_t uplo, size_t m, size_t n, rblas_complex32_t alpha,
+                    const rblas_complex32_t *a, size_t a_len, size_t lda, const rblas_complex32_t *b, size_t b_len,
+                    size_t ldb, rblas_complex32_t beta, rblas_complex32_t *c, size_t c_len, size_t ldc) {
+  return hesy_host<float>(false, side, uplo, m, n, alpha.re, alpha.im, (const float *)a, a_len, lda,
+                        (const float *)b, b_len, ldb, beta.re, beta.im, (float *)c, c_len, ldc);
+}
+
+int32_t rblas_csymm_dev(int32_t side, int32_t uplo, size_t m, size_t n, rblas_complex32_t alpha,
+                        const rblas_complex32_t *a, size_t a_len, size_t lda, const rblas_complex32_t *b,
+                        size_t b_len, size_t ldb, rblas_complex32_t beta, rblas_complex32_t *c, size_t c_len,
+                        size_t ldc, void *stream) {
+  return hesy_dev<float>(false, side, uplo, m, n, alpha.re, alpha.im, (const float *)a, a_len, lda,
+                       (const float *)b, b_len, ldb, beta.re, beta.im, (float *)c, c_len, ldc, stream);
+}
+
+int32_t rblas_zsymm(int32_t side, int32_t uplo, size_t m, size_t n, rblas_complex64_t alpha,
+                    const rblas_complex64_t *a, size_t a_len, size_t lda, const rblas_complex64_t *b, size_t b_len,
+                    size_t ldb, rblas_complex64_t beta, rblas_complex64_t *c, size_t c_len, size_t ldc) {
+  return hesy_host<double>(false, side, uplo, m, n, alpha.re, alpha.im, (const double *)a, a_len, lda,
+                        (const double *)b, b_len, ldb, beta.re, beta.im, (double *)c, c_len, ldc);
+}
+
+int32_t rblas_zsymm_dev(int32_t side, int32_t uplo, size_t m, size_t n, rblas_complex64_t alpha,
+                        const rblas_complex64_t *a, size_t a_len, size_t lda, const rblas_complex64_t *b,
+                        size_t b_len, size_t ldb, rblas_complex64_t beta, rblas_complex64_t *c, size_t c_len,
+                        size_t ldc, void *stream) {
+  return hesy_dev<double>(false, side, uplo, m, n, alpha.re, alpha.im, (const double *)a, a_len, lda,
+                       (const double *)b, b_len, ldb, beta.re, beta.im, (double *)c, c_len, ldc, stream);
+}
+
+int32_t rblas_chemm(int32_t side, int32_t uplo, size_t m, size_t n, rblas_complex32_t alpha,
+                    const rblas_complex32_t *a, size_t a_len, size_t lda, const rblas_complex32_t *b, size_t b_len,
+                    size_t ldb, rblas_complex32_t beta, rblas_complex32_t *c, size_t c_len, size_t ldc) {
+  return hesy_host<float>(true, side, uplo, m, n, alpha.re, alpha.im, (const float *)a, a_len, lda,
+                        (const float *)b, b_len, ldb, beta.re, beta.im, (float *)c, c_len, ldc);
+}
+
+int32_t rblas_chemm_dev(int32_t side, int32_t uplo, size_t m, size_t n, rblas_complex32_t alpha,
+                        const rblas_complex32_t *a, size_t a_len, size_t lda, const rblas_complex32_t *b,
+                        size_t b_len, size_t ldb, rblas_complex32_t beta, rblas_complex32_t *c, size_t c_len,
+                        size_t ldc, void *stream) {
+  return hesy_dev<float>(true, side, uplo, m, n, alpha.re, alpha.im, (const float *)a, a_len, lda,
+                       (const float *)b, b_len, ldb, beta.re, beta.im, (float *)c, c_len, ldc, stream);
+}
+
+int32_t rblas_zhemm(int32_t side, int32_t uplo, size_t m, size_t n, rblas_complex64_t alpha,
+                    const rblas_complex64_t *a, size_t a_len, size_t lda, const rblas_complex64_t *b, size_t b_len,
+                    size_t ldb, rblas_complex64_t beta, rblas_complex64_t *c, size_t c_len, size_t ldc) {
+  return hesy_host<double>(true, side, uplo, m, n, alpha.re, alpha.im, (const double *)a, a_len, lda,
+                        (const double *)b, b_len, ldb, beta.re, beta.im, (double *)c, c_len, ldc);
+}
+
+int32_t rblas_zhemm_dev(int32_t side, int32_t uplo, size_t m, size_t n, rblas_complex64_t alpha,
+                        const rblas_complex64_t *a, size_t a_len, size_t lda, const rblas_complex64_t *b,
+                        size_t b_len, size_t ldb, rblas_complex64_t beta, rblas_complex64_t *c, size_t c_len,
+                        size_t ldc, void *stream) {
+  return hesy_dev<double>(true, side, uplo, m, n, alpha.re, alpha.im, (const double *)a, a_len, lda,
+                       (const double *)b, b_len, ldb, beta.re, beta.im, (double *)c, c_len, ldc, stream);
 }
 
 }  // extern "C"

@@ -164,6 +164,122 @@ inline bool valid_compute(int32_t c) {
   return c == RBLAS_COMPUTE_F32 || c == RBLAS_COMPUTE_BF16X3 || c == RBLAS_COMPUTE_BF16X6;
 }
 
+// ---------------------------------------------------------------- symm
+// level3::symm (level3.rs:301-381): rblas builds the dense symmetric A from
+// its stored `uplo` triangle (full_symmetric: every read is a stored entry,
+// the largest index (q-1)(lda+1)), then calls gemm(ColMajor, N, N) with it
+// as the left (Side Left) or right (Side Right) operand. The same checks in
+// the same order: A's largest index, then that gemm's.
+template <typename T>
+Check symm_validate(int32_t side, int32_t uplo, size_t m, size_t n, T alpha, size_t a_len,
+                    size_t lda, const T *b, size_t b_len, size_t ldb, T beta, T *c, size_t c_len,
+                    size_t ldc, ColMajorCall<T> &cc) {
+  if ((side != RBLAS_LEFT && side != RBLAS_RIGHT) || (uplo != RBLAS_UPPER && uplo != RBLAS_LOWER)) {
+    Check bad;
+    bad.status = RBLAS_BAD_ENUM;
+    bad.err = {RBLAS_BAD_ENUM, 0, 0, 0};
+    bad.msg = "symm: enum argument out of range";
+    return bad;
+  }
+  const size_t q = side == RBLAS_LEFT ? m : n;
+  if (q > 0) {
+    size_t hi;
+    const bool fits = max_index(q, q, 1, lda, hi);
+    if (!fits || hi >= a_len)
+      return panic("index out of bounds: the len is " + std::to_string(a_len) +
+                   " but the index is " + (fits ? std::to_string(hi) : std::string("past usize")));
+  }
+  if (side == RBLAS_LEFT)
+    return normalize<T>(RBLAS_COL_MAJOR, 0, 0, m, n, q, alpha, nullptr, q * q, q, b, b_len, ldb,
+                        beta, c, c_len, ldc, cc);
+  return normalize<T>(RBLAS_COL_MAJOR, 0, 0, m, n, q, alpha, b, b_len, ldb, nullptr, q * q, q,
+                      beta, c, c_len, ldc, cc);
+}
+
+// Dense-A bytes at the front of a symm workspace (256-byte aligned).
+inline size_t sym_bytes(size_t q, size_t elem) { return (q * q * elem + 255) / 256 * 256; }
+
+// Enqueue a validated symm whose operands are in HBM: mirror A into `full`,
+// then the gemm on it.
+template <typename T>
+hipError_t symm_enqueue(const ColMajorCall<T> &cc, bool left, bool upper, const T *a, size_t lda,
+                        const T *b, size_t ldb, T *c, size_t ldc, T *full, void *gemm_ws,
+                        hipStream_t st) {
+  const size_t q = left ? cc.m : cc.n;
+  if (runs_product(cc)) {
+    hipError_t e = launch_sym_full<T>(a, (int64_t)lda, (int64_t)q, upper, false, false, full, st);
+    if (e != hipSuccess) return e;
+  }
+  if (left) return enqueue(cc, (const T *)full, q, b, ldb, c, ldc, 0, gemm_ws, st);
+  return enqueue(cc, b, ldb, (const T *)full, q, c, ldc, 0, gemm_ws, st);
+}
+
+template <typename T>
+int32_t symm_host(int32_t side, int32_t uplo, size_t m, size_t n, T alpha, const T *a,
+                  size_t a_len, size_t lda, const T *b, size_t b_len, size_t ldb, T beta, T *c,
+                  size_t c_len, size_t ldc) {
+  ColMajorCall<T> cc;
+  Check chk = symm_validate<T>(side, uplo, m, n, alpha, a_len, lda, b, b_len, ldb, beta, c, c_len,
+                               ldc, cc);
+  if (!chk.ok()) return fail(chk, nullptr);
+  if (m == 0 || n == 0) return RBLAS_OK;
+  const bool left = side == RBLAS_LEFT;
+  const size_t q = left ? m : n;
+  const bool prod = runs_product(cc);
+  CtxLease lease;
+  if (!lease.get()) return hip_fail(lease.status(), "stream create");
+  Ctx &ctx = *lease.get();
+  HostXfer x(ctx, sizeof(T), ((prod ? q * q + m * n : 0) + m * n) * sizeof(T));
+  Staged sa, sb, sc;
+  if (prod) {
+    sa.plan(q, q, lda, x.pinned());
+    sb.plan(m, n, ldb, x.pinned());
+  }
+  sc.plan(m, n, ldc, x.pinned());
+  const size_t full_b = prod ? sym_bytes(q, sizeof(T)) : 0;
+  RBLAS_HIP_TRY(ctx.ensure(0, sa.elems * sizeof(T)), "device buffer A");
+  RBLAS_HIP_TRY(ctx.ensure(1, sb.elems * sizeof(T)), "device buffer B");
+  RBLAS_HIP_TRY(ctx.ensure(2, sc.elems * sizeof(T)), "device buffer C");
+  RBLAS_HIP_TRY(ctx.ensure(3, full_b + workspace_bytes(cc, 0)), "device workspace");
+  RBLAS_HIP_TRY(x.reserve(sa, sb, sc), "pinned staging buffer");
+  T *da = (T *)ctx.buf[0], *db = (T *)ctx.buf[1], *dc = (T *)ctx.buf[2];
+  if (prod) {
+    RBLAS_HIP_TRY(x.in(sa, a, da), "copy A to device");
+    RBLAS_HIP_TRY(x.in(sb, b, db), "copy B to device");
+  }
+  RBLAS_HIP_TRY(x.in(sc, c, dc), "copy C to device");
+  RBLAS_HIP_TRY(symm_enqueue<T>(cc, left, uplo == RBLAS_UPPER, da, sa.dev_ld, db, sb.dev_ld, dc,
+                                sc.dev_ld, (T *)ctx.buf[3], (char *)ctx.buf[3] + full_b,
+                                ctx.stream),
+                "symm launch");
+  RBLAS_HIP_TRY(x.out(sc, dc, c), "copy C to host");
+  RBLAS_HIP_TRY(x.finish(), "symm");
+  return RBLAS_OK;
+}
+
+template <typename T>
+int32_t symm_dev(int32_t side, int32_t uplo, size_t m, size_t n, T alpha, const T *a, size_t a_len,
+                 size_t lda, const T *b, size_t b_len, size_t ldb, T beta, T *c, size_t c_len,
+                 size_t ldc, void *stream) {
+  ColMajorCall<T> cc;
+  Check chk = symm_validate<T>(side, uplo, m, n, alpha, a_len, lda, b, b_len, ldb, beta, c, c_len,
+                               ldc, cc);
+  if (!chk.ok()) return fail(chk, nullptr);
+  if (m == 0 || n == 0) return RBLAS_OK;
+  const bool left = side == RBLAS_LEFT;
+  const size_t q = left ? m : n;
+  hipStream_t st = (hipStream_t)stream;
+  const size_t full_b = runs_product(cc) ? sym_bytes(q, sizeof(T)) : 0;
+  const size_t wsb = full_b + workspace_bytes(cc, 0);
+  void *ws = nullptr;
+  if (wsb) RBLAS_HIP_TRY(pool_alloc(&ws, wsb, st), "symm workspace");
+  hipError_t e = symm_enqueue<T>(cc, left, uplo == RBLAS_UPPER, a, lda, b, ldb, c, ldc, (T *)ws,
+                                 ws ? (char *)ws + full_b : nullptr, st);
+  if (ws) hipFreeAsync(ws, st);
+  RBLAS_HIP_TRY(e, "symm launch");
+  return RBLAS_OK;
+}
+
 }  // namespace
 }  // namespace rblas
 
@@ -236,6 +352,35 @@ int32_t rblas_dgemm_dev(int32_t layout, int32_t transa, int32_t transb, size_t m
                         size_t c_len, size_t ldc, void *stream) {
   return gemm_dev_entry<double>(layout, transa, transb, m, n, k, alpha, a, a_len, lda, b, b_len,
                                 ldb, beta, c, c_len, ldc, stream);
+}
+
+int32_t rblas_ssymm(int32_t side, int32_t uplo, size_t m, size_t n, float alpha, const float *a,
+                    size_t a_len, size_t lda, const float *b, size_t b_len, size_t ldb, float beta,
+                    float *c, size_t c_len, size_t ldc) {
+  return symm_host<float>(side, uplo, m, n, alpha, a, a_len, lda, b, b_len, ldb, beta, c, c_len,
+                          ldc);
+}
+
+int32_t rblas_dsymm(int32_t side, int32_t uplo, size_t m, size_t n, double alpha, const double *a,
+                    size_t a_len, size_t lda, const double *b, size_t b_len, size_t ldb,
+                    double beta, double *c, size_t c_len, size_t ldc) {
+  return symm_host<double>(side, uplo, m, n, alpha, a, a_len, lda, b, b_len, ldb, beta, c, c_len,
+                           ldc);
+}
+
+int32_t rblas_ssymm_dev(int32_t side, int32_t uplo, size_t m, size_t n, float alpha,
+                        const float *a, size_t a_len, size_t lda, const float *b, size_t b_len,
+                        size_t ldb, float beta, float *c, size_t c_len, size_t ldc, void *stream) {
+  return symm_dev<float>(side, uplo, m, n, alpha, a, a_len, lda, b, b_len, ldb, beta, c, c_len,
+                         ldc, stream);
+}
+
+int32_t rblas_dsymm_dev(int32_t side, int32_t uplo, size_t m, size_t n, double alpha,
+                        const double *a, size_t a_len, size_t lda, const double *b, size_t b_len,
+                        size_t ldb, double beta, double *c, size_t c_len, size_t ldc,
+                        void *stream) {
+  return symm_dev<double>(side, uplo, m, n, alpha, a, a_len, lda, b, b_len, ldb, beta, c, c_len,
+                          ldc, stream);
 }
 
 }  // extern "C"

@@ -93,6 +93,15 @@ size_t cgemm_workspace_elems(int64_t m, int64_t n, int64_t k);
 template <typename T>
 hipError_t launch_cgemm(const CgemmProblem<T> &p, T *ws, hipStream_t stream);
 
+// out (q x q, ld q, dense) := the matrix whose `upper` (else lower) triangle
+// is stored in a (ld lda), mirrored across the diagonal; complex_elems:
+// interleaved complex elements (q x q of them), herm: the mirror is
+// conjugated and the diagonal's imaginary part is zero (Hermitian). Reads
+// only the stored triangle.
+template <typename T>
+hipError_t launch_sym_full(const T *a, int64_t lda, int64_t q, bool upper, bool complex_elems,
+                           bool herm, T *out, hipStream_t stream);
+
 // Triangular solve in place on B (m x n, column-major). `upper` is the
 // effective triangle of op(A) (uplo flipped by trans). Requires the whole
 // op(A) diagonal to be readable and m, n > 0 handled by the caller's checks.

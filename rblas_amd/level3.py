@@ -140,6 +140,25 @@ def syrk_dev(uplo, trans, n, k, alpha, a, lda, beta, c, ldc, stream=None) -> Non
     raise_status(st)
 
 
+def symm(side, uplo, m, n, alpha, a, lda, b, ldb, beta, c, ldc) -> None:
+    """C := alpha*A*B + beta*C (Left) or alpha*B*A + beta*C (Right), A
+    symmetric with its `uplo` triangle stored (level3.rs:301-381)."""
+    pre = _prefix(a, b, c)
+    st = getattr(_ffi.lib(), f"rblas_{pre}symm")(
+        int(side), int(uplo), m, n, _scalar(pre, alpha), a.ctypes.data, a.size, lda,
+        b.ctypes.data, b.size, ldb, _scalar(pre, beta), c.ctypes.data, c.size, ldc)
+    raise_status(st)
+
+
+def symm_dev(side, uplo, m, n, alpha, a, lda, b, ldb, beta, c, ldc, stream=None) -> None:
+    """symm on HBM-resident operands, enqueued on `stream`."""
+    pre = _dev_prefix(a, b, c)
+    st = getattr(_ffi.lib(), f"rblas_{pre}symm_dev")(
+        int(side), int(uplo), m, n, _scalar(pre, alpha), a.ptr, a.size, lda, b.ptr, b.size, ldb,
+        _scalar(pre, beta), c.ptr, c.size, ldc, stream.handle if stream is not None else None)
+    raise_status(st)
+
+
 def trsm(side, uplo, trans, diag, m, n, alpha, a, lda, b, ldb) -> None:
     """Solve op(A) X = alpha B or X op(A) = alpha B in place in b
     (level3.rs:390-520)."""
