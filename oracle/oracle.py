@@ -6,6 +6,8 @@ algorithms, see oracle.cpp) plus restatements of rblas's argument checks:
 
   check_matrix / try_gemm_check   level3.rs:68-140
   gemm_panics                     level3.rs:143-222 + kernel/gemm.rs:198-205,322-338
+  syrk_panics / trsm_panics       level3.rs:236-294 / 390-520 (index walks)
+  symm / hesy and their panics    level3.rs:301-381, complex3.rs:127-247
 
 All arrays are numpy, 1-D "slices" exactly as rblas takes them.
 """
