@@ -237,6 +237,32 @@ int32_t rblas_dtrsm_dev(int32_t side, int32_t uplo, int32_t trans, int32_t diag,
                         size_t a_len, size_t lda, double *b, size_t b_len,
                         size_t ldb, void *stream);
 
+/* ------------------------------------------------------------ complex trsm
+ * op(A) X = alpha B (side Left) or X op(A) = alpha B (Right), in place in B,
+ * complex, column-major; op = None, Trans or ConjTrans (conjugating).
+ * Replaces `complex3::trsm` (complex3.rs:252-349): the same checks as the
+ * real trsm (nothing when m or n is 0; RBLAS_PANIC at B's largest index
+ * (m-1) + (n-1)*ldb, then op(A)'s diagonal (q-1)*(lda+1)); RBLAS_BAD_ARG for
+ * ldb < m with n > 1. alpha != 1 pre-scales B; NonUnit divides by
+ * rblas's cdiv (s conj(a_ii) / norm_sqr(a_ii)). Computed by recursive
+ * halving: 4M complex gemm updates and 64-wide substitution leaves in f64.
+ * Lengths and leading dimensions count complex elements. */
+int32_t rblas_ctrsm(int32_t side, int32_t uplo, int32_t trans, int32_t diag, size_t m,
+                    size_t n, rblas_complex32_t alpha, const rblas_complex32_t *a,
+                    size_t a_len, size_t lda, rblas_complex32_t *b, size_t b_len, size_t ldb);
+int32_t rblas_ztrsm(int32_t side, int32_t uplo, int32_t trans, int32_t diag, size_t m,
+                    size_t n, rblas_complex64_t alpha, const rblas_complex64_t *a,
+                    size_t a_len, size_t lda, rblas_complex64_t *b, size_t b_len, size_t ldb);
+/* HBM-resident forms (no rblas counterpart). */
+int32_t rblas_ctrsm_dev(int32_t side, int32_t uplo, int32_t trans, int32_t diag, size_t m,
+                        size_t n, rblas_complex32_t alpha, const rblas_complex32_t *a,
+                        size_t a_len, size_t lda, rblas_complex32_t *b, size_t b_len,
+                        size_t ldb, void *stream);
+int32_t rblas_ztrsm_dev(int32_t side, int32_t uplo, int32_t trans, int32_t diag, size_t m,
+                        size_t n, rblas_complex64_t alpha, const rblas_complex64_t *a,
+                        size_t a_len, size_t lda, rblas_complex64_t *b, size_t b_len,
+                        size_t ldb, void *stream);
+
 /* -------------------------------------------------------------- symm/hemm
  * C := alpha * A * B + beta * C (side Left, A m x m) or alpha * B * A +
  * beta * C (side Right, A n x n), B and C m x n, column-major, where A is

@@ -394,6 +394,87 @@ void cmul(T ar, T ai, T br, T bi, T &re, T &im) {
   im = std::fma(ar, bi, u);
 }
 
+// complex3.rs:252-349 — complex trsm by substitution: alpha pre-scale when
+// alpha != 1 (complex multiply), s = s - aop*x per component, finish =
+// cdiv(s, a_ii) = s * conj(a_ii) / norm_sqr(a_ii) with norm_sqr =
+// fma(re, re, im*im) (complex.rs:68-70, complex3.rs:344-349); ConjTrans
+// conjugates op(A) (conj: im = 0 - im). Interleaved {re, im} storage.
+template <typename T>
+void ctrsm(int side, int uplo, int trans, int diag, size_t m, size_t n, T alr, T ali, const T *a,
+           size_t lda, T *b, size_t ldb) {
+  if (!(alr == T(1) && ali == T(0)))
+    for (size_t j = 0; j < n; ++j)
+      for (size_t i = 0; i < m; ++i) {
+        T *x = b + 2 * (i + j * ldb);
+        T re, im;
+        cmul(alr, ali, x[0], x[1], re, im);
+        x[0] = re;
+        x[1] = im;
+      }
+  const bool tr = trans != 0, cj = trans == 2;
+  auto aop = [&](size_t r, size_t c, T &re, T &im) {
+    const size_t idx = tr ? c + r * lda : r + c * lda;
+    re = a[2 * idx];
+    im = cj ? T(0) - a[2 * idx + 1] : a[2 * idx + 1];
+  };
+  const bool upper = (uplo == 0) != tr;
+  const bool unit = diag == 1;
+  auto finish = [&](T sr, T si, T ar, T ai, T *x) {
+    if (unit) {
+      x[0] = sr;
+      x[1] = si;
+      return;
+    }
+    const T d = std::fma(ar, ar, ai * ai);
+    T nr, ni;
+    cmul(sr, si, ar, T(0) - ai, nr, ni);
+    x[0] = nr / d;
+    x[1] = ni / d;
+  };
+  auto step = [&](T &sr, T &si, T pr_a, T pi_a, T pr_b, T pi_b) {
+    T re, im;
+    cmul(pr_a, pi_a, pr_b, pi_b, re, im);
+    sr = sr - re;
+    si = si - im;
+  };
+  if (side == 0) {
+    for (size_t j = 0; j < n; ++j) {
+      T *col = b + 2 * j * ldb;
+      for (size_t t = 0; t < m; ++t) {
+        const size_t ii = upper ? m - 1 - t : t;
+        T sr = col[2 * ii], si = col[2 * ii + 1];
+        const size_t k0 = upper ? ii + 1 : 0, k1 = upper ? m : ii;
+        for (size_t kk = k0; kk < k1; ++kk) {
+          T ar, ai;
+          aop(ii, kk, ar, ai);
+          step(sr, si, ar, ai, col[2 * kk], col[2 * kk + 1]);
+        }
+        T dr, di;
+        aop(ii, ii, dr, di);
+        finish(sr, si, dr, di, col + 2 * ii);
+      }
+    }
+  } else {
+    for (size_t i = 0; i < m; ++i) {
+      for (size_t t = 0; t < n; ++t) {
+        const size_t j = upper ? t : n - 1 - t;
+        T *x = b + 2 * (i + j * ldb);
+        T sr = x[0], si = x[1];
+        const size_t k0 = upper ? 0 : j + 1, k1 = upper ? j : n;
+        for (size_t kk = k0; kk < k1; ++kk) {
+          T ar, ai;
+          aop(kk, j, ar, ai);
+          const T *y = b + 2 * (i + kk * ldb);
+          step(sr, si, y[0], y[1], ar, ai);
+        }
+        T dr, di;
+        aop(j, j, dr, di);
+        finish(sr, si, dr, di, x);
+      }
+    }
+  }
+}
+
 // complex3.rs:27-125 — 4M complex gemm on split planes (column-major).
 // Interleaved {re, im} storage; trans 2 = ConjTrans conjugates.
 template <typename T>
@@ -571,6 +652,14 @@ void oracle_zgemm(int ta, int tb, size_t m, size_t n, size_t k, double alr, doub
                   const double *a, size_t lda, const double *b, size_t ldb, double ber,
                   double bei, double *c, size_t ldc, int threads) {
   cgemm<double>(ta, tb, m, n, k, alr, ali, a, lda, b, ldb, ber, bei, c, ldc, threads);
+}
+void oracle_ctrsm(int side, int uplo, int trans, int diag, size_t m, size_t n, float alr,
+                  float ali, const float *a, size_t lda, float *b, size_t ldb) {
+  ctrsm<float>(side, uplo, trans, diag, m, n, alr, ali, a, lda, b, ldb);
+}
+void oracle_ztrsm(int side, int uplo, int trans, int diag, size_t m, size_t n, double alr,
+                  double ali, const double *a, size_t lda, double *b, size_t ldb) {
+  ctrsm<double>(side, uplo, trans, diag, m, n, alr, ali, a, lda, b, ldb);
 }
 float oracle_sdot(size_t n, const float *x, size_t incx, const float *y, size_t incy) {
   return dot<float>(n, x, incx, y, incy);

@@ -8,6 +8,7 @@ algorithms, see oracle.cpp) plus restatements of rblas's argument checks:
   gemm_panics                     level3.rs:143-222 + kernel/gemm.rs:198-205,322-338
   syrk_panics / trsm_panics       level3.rs:236-294 / 390-520 (index walks)
   symm / hesy and their panics    level3.rs:301-381, complex3.rs:127-247
+  ctrsm (C++)                     complex3.rs:252-349 (panics: trsm_panics, same loops)
 
 All arrays are numpy, 1-D "slices" exactly as rblas takes them.
 """
@@ -57,6 +58,10 @@ def lib() -> C.CDLL:
         for name, t, tp in (("oracle_cgemm", C.c_float, f32p), ("oracle_zgemm", C.c_double, f64p)):
             fn = getattr(_lib, name)
             fn.argtypes = [i, i, sz, sz, sz, t, t, tp, sz, tp, sz, t, t, tp, sz, i]
+            fn.restype = None
+        for name, t, tp in (("oracle_ctrsm", C.c_float, f32p), ("oracle_ztrsm", C.c_double, f64p)):
+            fn = getattr(_lib, name)
+            fn.argtypes = [i, i, i, i, sz, sz, t, t, tp, sz, tp, sz]
             fn.restype = None
         for pre, t, tp in (("s", C.c_float, f32p), ("d", C.c_double, f64p)):
             fn = getattr(_lib, f"oracle_{pre}dot")
@@ -110,6 +115,16 @@ def cgemm(ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, threads=1):
     av, bv, cv = (x.view(real) for x in (a, b, c))
     fn(ta, tb, m, n, k, complex(alpha).real, complex(alpha).imag, _ptr(av), lda, _ptr(bv), ldb,
        complex(beta).real, complex(beta).imag, _ptr(cv), ldc, threads)
+
+
+def ctrsm(side, uplo, trans, diag, m, n, alpha, a, lda, b, ldb):
+    """complex3::trsm (complex3.rs:252-349) restated; in place on b
+    (complex64 / complex128 arrays)."""
+    real = np.float32 if b.dtype == np.complex64 else np.float64
+    fn = lib().oracle_ctrsm if real == np.float32 else lib().oracle_ztrsm
+    av, bv = (x.view(real) for x in (a, b))
+    fn(side, uplo, trans, diag, m, n, complex(alpha).real, complex(alpha).imag, _ptr(av), lda,
+       _ptr(bv), ldb)
 
 
 def dot(n, x, incx, y, incy):

@@ -136,6 +136,10 @@ SIGNATURES: dict[str, tuple[list, object]] = {
                          sz, vp], i32),
     "rblas_zhemm_dev": ([i32, i32, sz, sz, Complex64, vp, sz, sz, vp, sz, sz, Complex64, vp, sz,
                          sz, vp], i32),
+    "rblas_ctrsm": ([i32, i32, i32, i32, sz, sz, Complex32, vp, sz, sz, vp, sz, sz], i32),
+    "rblas_ztrsm": ([i32, i32, i32, i32, sz, sz, Complex64, vp, sz, sz, vp, sz, sz], i32),
+    "rblas_ctrsm_dev": ([i32, i32, i32, i32, sz, sz, Complex32, vp, sz, sz, vp, sz, sz, vp], i32),
+    "rblas_ztrsm_dev": ([i32, i32, i32, i32, sz, sz, Complex64, vp, sz, sz, vp, sz, sz, vp], i32),
     "rblas_sdot": ([sz, vp, sz, sz, vp, sz, sz, C.POINTER(f32)], i32),
     "rblas_ddot": ([sz, vp, sz, sz, vp, sz, sz, C.POINTER(f64)], i32),
     "rblas_snrm2": ([sz, vp, sz, sz, C.POINTER(f32)], i32),

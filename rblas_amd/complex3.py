@@ -86,3 +86,25 @@ def symm_dev(side, uplo, m, n, alpha, a, lda, b, ldb, beta, c, ldc, stream=None)
 def hemm_dev(side, uplo, m, n, alpha, a, lda, b, ldb, beta, c, ldc, stream=None) -> None:
     """hemm on HBM-resident complex operands (device.DeviceArray)."""
     _hesy_dev("hemm", side, uplo, m, n, alpha, a, lda, b, ldb, beta, c, ldc, stream)
+
+
+def trsm(side, uplo, trans, diag, m, n, alpha, a, lda, b, ldb) -> None:
+    """Solve op(A) X = alpha B or X op(A) = alpha B in place in b, op with
+    conjugation for ConjTrans (complex3.rs:252-349)."""
+    pre, cs = _kind(a, b)
+    al = complex(alpha)
+    st = getattr(_ffi.lib(), f"rblas_{pre}trsm")(
+        int(side), int(uplo), int(trans), int(diag), m, n, cs(al.real, al.imag), a.ctypes.data,
+        a.size, lda, b.ctypes.data, b.size, ldb)
+    raise_status(st)
+
+
+def trsm_dev(side, uplo, trans, diag, m, n, alpha, a, lda, b, ldb, stream=None) -> None:
+    """Complex trsm on HBM-resident operands (device.DeviceArray)."""
+    pre = {np.dtype(np.complex64): "c", np.dtype(np.complex128): "z"}[b.dtype]
+    cs = _ffi.Complex32 if pre == "c" else _ffi.Complex64
+    al = complex(alpha)
+    st = getattr(_ffi.lib(), f"rblas_{pre}trsm_dev")(
+        int(side), int(uplo), int(trans), int(diag), m, n, cs(al.real, al.imag), a.ptr, a.size,
+        lda, b.ptr, b.size, ldb, stream.handle if stream is not None else None)
+    raise_status(st)

@@ -4,6 +4,7 @@
 
 #include "kernels.hpp"
 #include "runtime.hpp"
+#include "trsm_host.hpp"
 
 namespace rblas {
 namespace {
@@ -221,6 +222,82 @@ int32_t hesy_dev(bool herm, int32_t side, int32_t uplo, size_t m, size_t n, T ar
   return RBLAS_OK;
 }
 
+// ---------------------------------------------------------------- trsm
+// complex3::trsm (complex3.rs:252-349) has the loops of level3::trsm, so its
+// panics are at the same indices (lengths in complex elements): nothing
+// when m or n is 0; B's largest index (m-1) + (n-1)*ldb, then op(A)'s
+// diagonal (q-1)*(lda+1). The 4M updates add their own size limit.
+template <typename T>
+Check ctrsm_validate(int32_t side, int32_t uplo, int32_t trans, int32_t diag, size_t m, size_t n,
+                     size_t a_len, size_t lda, size_t b_len, size_t ldb) {
+  Check c = trsm_validate<T>(side, uplo, trans, diag, m, n, a_len, lda, b_len, ldb);
+  if (!c.ok() || m == 0 || n == 0) return c;
+  const size_t q = side == RBLAS_LEFT ? m : n;
+  if ((2 * q * 128 + 160) * sizeof(T) >= 0x7fffffe0ull)
+    return bad_arg("complex trsm: order too large for the device path");
+  return Check{};
+}
+
+template <typename T>
+CtrsmProblem<T> ctrsm_problem(int32_t side, int32_t uplo, int32_t trans, int32_t diag, size_t m,
+                              size_t n, T ar, T ai, const T *a, size_t lda, T *b, size_t ldb) {
+  return CtrsmProblem<T>{side == RBLAS_LEFT, uplo == RBLAS_UPPER, trans, diag == RBLAS_UNIT,
+                         (int64_t)m, (int64_t)n, ar, ai, a, (int64_t)lda, b, (int64_t)ldb};
+}
+
+template <typename T>
+int32_t ctrsm_host(int32_t side, int32_t uplo, int32_t trans, int32_t diag, size_t m, size_t n,
+                   T ar, T ai, const T *a, size_t a_len, size_t lda, T *b, size_t b_len,
+                   size_t ldb) {
+  Check chk = ctrsm_validate<T>(side, uplo, trans, diag, m, n, a_len, lda, b_len, ldb);
+  if (!chk.ok()) return fail(chk, nullptr);
+  if (m == 0 || n == 0) return RBLAS_OK;
+  const bool left = side == RBLAS_LEFT;
+  const size_t q = left ? m : n;
+  CtxLease lease;
+  if (!lease.get()) return hip_fail(lease.status(), "stream create");
+  Ctx &ctx = *lease.get();
+  const size_t cs = 2 * sizeof(T);
+  HostXfer x(ctx, cs, (q * q + m * n) * cs);
+  Staged sa, sb;
+  sa.plan(q, q, lda, x.pinned());
+  sb.plan(m, n, ldb, x.pinned());
+  RBLAS_HIP_TRY(ctx.ensure(0, sa.elems * cs), "device buffer A");
+  RBLAS_HIP_TRY(ctx.ensure(2, sb.elems * cs), "device buffer B");
+  RBLAS_HIP_TRY(ctx.ensure(3, ctrsm_workspace_elems((int64_t)m, (int64_t)n, left) * sizeof(T)),
+                "device workspace");
+  RBLAS_HIP_TRY(x.reserve(sa, sb), "pinned staging buffer");
+  RBLAS_HIP_TRY(x.in(sa, a, ctx.buf[0]), "copy A");
+  RBLAS_HIP_TRY(x.in(sb, b, ctx.buf[2]), "copy B");
+  const CtrsmProblem<T> p = ctrsm_problem<T>(side, uplo, trans, diag, m, n, ar, ai,
+                                             (const T *)ctx.buf[0], sa.dev_ld, (T *)ctx.buf[2],
+                                             sb.dev_ld);
+  RBLAS_HIP_TRY(launch_ctrsm<T>(p, (T *)ctx.buf[3], ctx.stream), "complex trsm launch");
+  RBLAS_HIP_TRY(x.out(sb, ctx.buf[2], b), "copy B back");
+  RBLAS_HIP_TRY(x.finish(), "complex trsm");
+  return RBLAS_OK;
+}
+
+template <typename T>
+int32_t ctrsm_dev(int32_t side, int32_t uplo, int32_t trans, int32_t diag, size_t m, size_t n,
+                  T ar, T ai, const T *a, size_t a_len, size_t lda, T *b, size_t b_len,
+                  size_t ldb, void *stream) {
+  Check chk = ctrsm_validate<T>(side, uplo, trans, diag, m, n, a_len, lda, b_len, ldb);
+  if (!chk.ok()) return fail(chk, nullptr);
+  if (m == 0 || n == 0) return RBLAS_OK;
+  hipStream_t st = (hipStream_t)stream;
+  void *ws = nullptr;
+  RBLAS_HIP_TRY(pool_alloc(&ws, ctrsm_workspace_elems((int64_t)m, (int64_t)n,
+                                                      side == RBLAS_LEFT) * sizeof(T), st),
+                "complex trsm workspace");
+  const CtrsmProblem<T> p = ctrsm_problem<T>(side, uplo, trans, diag, m, n, ar, ai, a, lda, b,
+                                             ldb);
+  hipError_t e = launch_ctrsm<T>(p, (T *)ws, st);
+  hipFreeAsync(ws, st);
+  RBLAS_HIP_TRY(e, "complex trsm launch");
+  return RBLAS_OK;
+}
+
 }  // namespace
 }  // namespace rblas
 
@@ -324,6 +401,34 @@ int32_t rblas_zhemm_dev(int32_t side, int32_t uplo, size_t m, size_t n, rblas_co
                         size_t ldc, void *stream) {
   return hesy_dev<double>(true, side, uplo, m, n, alpha.re, alpha.im, (const double *)a, a_len, lda,
                        (const double *)b, b_len, ldb, beta.re, beta.im, (double *)c, c_len, ldc, stream);
+}
+
+int32_t rblas_ctrsm(int32_t side, int32_t uplo, int32_t trans, int32_t diag, size_t m,
+                    size_t n, rblas_complex32_t alpha, const rblas_complex32_t *a, size_t a_len, size_t lda,
+                    rblas_complex32_t *b, size_t b_len, size_t ldb) {
+  return ctrsm_host<float>(side, uplo, trans, diag, m, n, alpha.re, alpha.im, (const float *)a, a_len,
+                         lda, (float *)b, b_len, ldb);
+}
+
+int32_t rblas_ctrsm_dev(int32_t side, int32_t uplo, int32_t trans, int32_t diag, size_t m,
+                        size_t n, rblas_complex32_t alpha, const rblas_complex32_t *a, size_t a_len,
+                        size_t lda, rblas_complex32_t *b, size_t b_len, size_t ldb, void *stream) {
+  return ctrsm_dev<float>(side, uplo, trans, diag, m, n, alpha.re, alpha.im, (const float *)a, a_len,
+                        lda, (float *)b, b_len, ldb, stream);
+}
+
+int32_t rblas_ztrsm(int32_t side, int32_t uplo, int32_t trans, int32_t diag, size_t m,
+                    size_t n, rblas_complex64_t alpha, const rblas_complex64_t *a, size_t a_len, size_t lda,
+                    rblas_complex64_t *b, size_t b_len, size_t ldb) {
+  return ctrsm_host<double>(side, uplo, trans, diag, m, n, alpha.re, alpha.im, (const double *)a, a_len,
+                         lda, (double *)b, b_len, ldb);
+}
+
+int32_t rblas_ztrsm_dev(int32_t side, int32_t uplo, int32_t trans, int32_t diag, size_t m,
+                        size_t n, rblas_complex64_t alpha, const rblas_complex64_t *a, size_t a_len,
+                        size_t lda, rblas_complex64_t *b, size_t b_len, size_t ldb, void *stream) {
+  return ctrsm_dev<double>(side, uplo, trans, diag, m, n, alpha.re, alpha.im, (const double *)a, a_len,
+                        lda, (double *)b, b_len, ldb, stream);
 }
 
 }  // extern "C"

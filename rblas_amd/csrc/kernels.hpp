@@ -93,6 +93,26 @@ size_t cgemm_workspace_elems(int64_t m, int64_t n, int64_t k);
 template <typename T>
 hipError_t launch_cgemm(const CgemmProblem<T> &p, T *ws, hipStream_t stream);
 
+// Complex trsm (interleaved complex B and A): op(A) X = alpha B (left) or
+// X op(A) = alpha B, in place; `upper` is the stored triangle (uplo Upper),
+// trans 0 / 1 / 2 = None / Trans / ConjTrans.
+template <typename T>
+struct CtrsmProblem {
+  bool left, upper;
+  int trans;
+  bool unit;
+  int64_t m, n;
+  T alpha_re, alpha_im;
+  const T *a;
+  int64_t lda;
+  T *b;
+  int64_t ldb;
+};
+// Workspace of launch_ctrsm: ctrsm_workspace_elems(m, n, left) elements of T.
+size_t ctrsm_workspace_elems(int64_t m, int64_t n, bool left);
+template <typename T>
+hipError_t launch_ctrsm(const CtrsmProblem<T> &p, T *ws, hipStream_t stream);
+
 // out (q x q, ld q, dense) := the matrix whose `upper` (else lower) triangle
 // is stored in a (ld lda), mirrored across the diagonal; complex_elems:
 // interleaved complex elements (q x q of them), herm: the mirror is
