@@ -414,6 +414,34 @@ def trsm_single():
     return out
 
 
+def ctrsm_single():
+    """Complex trsm on resident operands, q = nrhs = 4096: ZTRSM and CTRSM
+    Left / Lower / NoTrans, and ZTRSM Right / Upper / ConjTrans (TFLOP/s in
+    4 q^2 nrhs real flops)."""
+    from rblas_amd import complex3
+    from rblas_amd.device import DeviceArray, Stream
+    rng = np.random.default_rng(12)
+    s = Stream()
+    out = {}
+    q = 4096
+    full = (rng.standard_normal((q, q)) + 1j * rng.standard_normal((q, q))) / np.sqrt(q)
+    np.fill_diagonal(full, 2.0 + np.abs(rng.standard_normal(q)))
+    b = rng.standard_normal(q * q) + 1j * rng.standard_normal(q * q)
+    for tag, side, uplo, trans, a, ct in (
+            ("z_llnn", 0, 1, 0, np.tril(full), np.complex128),
+            ("c_llnn", 0, 1, 0, np.tril(full), np.complex64),
+            ("z_ruct", 1, 0, 2, np.triu(full), np.complex128)):
+        da = DeviceArray.from_host(a.T.reshape(-1).astype(ct), s)
+        db = DeviceArray.from_host(b.astype(ct), s)
+        ms = timed(lambda: complex3.trsm_dev(side, uplo, trans, 0, q, q, 1.0, da, q, db, q, s),
+                   s, 5, 2)
+        out[f"{tag}_{q}"] = {"ms": round(ms, 3), "tflops": round(4.0 * q ** 3 / ms / 1e9, 2)}
+        da.free()
+        db.free()
+    s.close()
+    return out
+
+
 def serving(count):
     from rblas_amd import serve
     closed = serve.workload(count, 0.0)
@@ -508,6 +536,7 @@ ONLY = {
     "grouped_trsm_f64": lambda a: grouped_trsm(),
     "trsm": lambda a: trsm_single(),
     "symm_hemm": lambda a: symm_hemm(),
+    "ctrsm": lambda a: ctrsm_single(),
     "serve": lambda a: serving(a.serve_requests),
     "c64_gemm": lambda a: c64_gemm(),
     "sgemm_skinny": lambda a: skinny_sgemm_tflops(),
@@ -591,6 +620,8 @@ def main():
         comp["trsm"] = trsm_single()
         log("symm / hemm ...")
         comp["symm_hemm"] = symm_hemm()
+        log("complex trsm ...")
+        comp["ctrsm"] = ctrsm_single()
         log("grouped trsm ...")
         comp["grouped_trsm_f64"] = grouped_trsm()
         log("serving ...")

@@ -7,12 +7,11 @@
 //      comes first in substitution order, subtract its contribution from the
 //      other half with one 4M complex gemm (alpha -1, beta 1), solve the other
 //      half.
-//   3. A range of up to 64 is solved by substitution, as rblas does it
+//   3. A range of up to 64 is solved by substitution with rblas's arithmetic
 //      (s = s - a*x, then cdiv(s, a_ii) = s conj(a_ii) / norm_sqr(a_ii)), in
-//      f64: one thread per right-hand side (a column of B for Left, a row for
-//      Right), with the block of op(A) and the right-hand sides in LDS. A
-//      non-finite B entry reaches exactly the entries rblas's substitution
-//      reaches.
+//      f64, 16 lanes per right-hand side (a column of B for Left, a row for
+//      Right) and the block of op(A) in LDS. A non-finite B entry reaches
+//      exactly the entries rblas's substitution reaches.
 #include <cmath>
 
 #include "kernels.hpp"
@@ -29,6 +28,12 @@ struct Cd {
 // complex.rs:101-111: (a+bi)(c+di) with rblas's fma pattern.
 __device__ __forceinline__ Cd cmul(Cd a, Cd b) {
   return Cd{__builtin_fma(a.re, b.re, 0.0 - a.im * b.im), __builtin_fma(a.re, b.im, a.im * b.re)};
+}
+
+// Componentwise select: a conditional on two Cd lvalues would select
+// between their addresses and force both into memory.
+__device__ __forceinline__ Cd csel(bool c, Cd a, Cd b) {
+  return Cd{c ? a.re : b.re, c ? a.im : b.im};
 }
 
 // complex3.rs:344-349: s * conj(d) / (fma(d.re, d.re, d.im * d.im)).
@@ -51,90 +56,165 @@ __global__ void cscale_kernel(int64_t m, int64_t n, T ar, T ai, T *b, int64_t ld
   }
 }
 
-// op(A)(r, c), conjugated for ConjTrans.
-template <typename T>
-__device__ __forceinline__ Cd aop(const T *a, int64_t lda, int trans, int64_t r, int64_t c) {
-  const int64_t idx = trans ? c + r * lda : r + c * lda;
-  return Cd{(double)a[2 * idx], trans == 2 ? 0.0 - (double)a[2 * idx + 1] : (double)a[2 * idx + 1]};
+// Leaves: one 16-lane DPP row per right-hand side (a column of B for Left,
+// a row for Right), each lane holding entries r = g + 16 jj (g its lane in
+// the row, jj < 4) in registers. The solve is right-looking: at step i the
+// lane that holds entry i divides it by the diagonal, the quotient x_i is
+// broadcast across the row, and every later entry r subtracts
+// coef(i, r) x_i. An entry receives its subtractions in rblas's order (k
+// ascending) in a forward solve and in descending k in a backward one; the
+// entries a product with a non-finite x_i reaches are the ones rblas's sums
+// reach (every k in the triangle, zero coefficients included). Steps are unrolled so that the
+// owning lane and register are compile-time constants.
+constexpr int kRowLanes = 16;                 // lanes per right-hand side (a DPP row)
+constexpr int kPer = kNB / kRowLanes;         // entries held per lane
+constexpr int kLeafThreads = 256;
+constexpr int kRhsPerBlock = kLeafThreads / kRowLanes;
+
+// Value of lane S of each row of 16 lanes, for every lane of the row.
+template <int S>
+__device__ __forceinline__ int row_bcast_imm(int v) {
+  return __builtin_amdgcn_update_dpp(0, v, 0x150 + S, 0xf, 0xf, false);
 }
 
-// Diagonal block of op(A) at (lo, lo), nb x nb, into As[r][c] (only the
-// effective triangle is read).
-template <typename T>
-__device__ __forceinline__ void load_block(const T *a, int64_t lda, int trans, bool upper,
-                                           int64_t lo, int nb, Cd (*As)[kNB + 1]) {
-  for (int e = threadIdx.x; e < kNB * kNB; e += blockDim.x) {
-    const int r = e % kNB, c = e / kNB;
-    const bool in = r < nb && c < nb && (upper ? r <= c : r >= c);
-    As[r][c] = in ? aop(a, lda, trans, lo + r, lo + c) : Cd{0.0, 0.0};
+template <int S>
+__device__ __forceinline__ double row_bcast(double v) {
+  const unsigned long long u = (unsigned long long)__double_as_longlong(v);
+  const unsigned lo = (unsigned)row_bcast_imm<S>((int)(unsigned)u);
+  const unsigned hi = (unsigned)row_bcast_imm<S>((int)(unsigned)(u >> 32));
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+// The four entries a lane holds, addressed by compile-time index only so
+// that they stay in registers.
+struct Held {
+  Cd v0, v1, v2, v3;
+  template <int J>
+  __device__ __forceinline__ Cd &at() {
+    if constexpr (J == 0) return v0;
+    else if constexpr (J == 1) return v1;
+    else if constexpr (J == 2) return v2;
+    else return v3;
+  }
+};
+static_assert(kPer == 4, "Held keeps four entries per lane");
+
+// Entry r = g + 16 J after x_i is known.
+template <bool LEFT, bool FORWARD, int I, int J>
+__device__ __forceinline__ void leaf_update(const Cd (*U)[kNB], int g, Cd xi, Held &h) {
+  constexpr int i = FORWARD ? I : kNB - 1 - I;
+  constexpr int jo = i / kRowLanes;
+  if constexpr (FORWARD ? J >= jo : J <= jo) {
+    const int r = g + kRowLanes * J;
+    const Cd c = U[i][r];
+    const Cd p = LEFT ? cmul(c, xi) : cmul(xi, c);
+    Cd &e = h.at<J>();
+    const Cd upd{e.re - p.re, e.im - p.im};
+    if constexpr (J == jo) {
+      const bool later = FORWARD ? r > i : r < i;
+      e = csel(later, upd, csel(r == i, xi, e));
+    } else {
+      e = upd;
+    }
   }
 }
 
-// Left: rows [lo, lo + nb) of B, column j = blockIdx.x * 64 + thread.
-template <typename T>
-__global__ __launch_bounds__(kNB) void ctrsm_leaf_left(const T *a, int64_t lda, int trans,
-                                                       int upper, int unit, int64_t lo, int nb,
-                                                       int64_t n, T *b, int64_t ldb) {
-  __shared__ Cd As[kNB][kNB + 1];
-  __shared__ Cd X[kNB][kNB];  // X[row][thread]
-  load_block(a, lda, trans, upper, lo, nb, As);
-  const int t = threadIdx.x;
-  const int64_t j = blockIdx.x * (int64_t)kNB + t;
-  const bool on = j < n;
-  T *col = b + 2 * (lo + j * ldb);
-  if (on)
-    for (int r = 0; r < nb; ++r) X[r][t] = Cd{(double)col[2 * r], (double)col[2 * r + 1]};
+// Step I of the solve in substitution order: entry i = FORWARD ? I : 63 - I.
+template <bool LEFT, bool FORWARD, int I>
+__device__ __forceinline__ void leaf_step(const Cd (*U)[kNB], int nb, int unit, int g,
+                                          Held &h) {
+  constexpr int i = FORWARD ? I : kNB - 1 - I;
+  constexpr int go = i % kRowLanes, jo = i / kRowLanes;
+  if (i >= nb) return;  // uniform: the block is shorter than 64
+  const Cd v = h.at<jo>();
+  Cd xi = v;
+  if (!unit) xi = cdiv(v, U[i][i]);
+  xi.re = row_bcast<go>(xi.re);
+  xi.im = row_bcast<go>(xi.im);
+  leaf_update<LEFT, FORWARD, I, 0>(U, g, xi, h);
+  leaf_update<LEFT, FORWARD, I, 1>(U, g, xi, h);
+  leaf_update<LEFT, FORWARD, I, 2>(U, g, xi, h);
+  leaf_update<LEFT, FORWARD, I, 3>(U, g, xi, h);
+}
+
+template <bool LEFT, bool FORWARD, int I>
+struct LeafRun {
+  __device__ __forceinline__ static void go(const Cd (*U)[kNB], int nb, int unit, int g,
+                                            Held &h) {
+    leaf_step<LEFT, FORWARD, I>(U, nb, unit, g, h);
+    LeafRun<LEFT, FORWARD, I + 1>::go(U, nb, unit, g, h);
+  }
+};
+template <bool LEFT, bool FORWARD>
+struct LeafRun<LEFT, FORWARD, kNB> {
+  __device__ __forceinline__ static void go(const Cd (*)[kNB], int, int, int, Held &) {}
+};
+
+// B entry r of right-hand side `rhs`: row lo + r of column rhs (Left) or
+// column lo + r of row rhs (Right).
+template <bool LEFT>
+__device__ __forceinline__ int64_t b_index(int64_t lo, int r, int64_t rhs, int64_t ldb) {
+  return LEFT ? lo + r + rhs * ldb : rhs + (lo + r) * ldb;
+}
+
+template <typename T, bool LEFT, int J>
+__device__ __forceinline__ void leaf_load(const T *b, int64_t lo, int nb, int64_t rhs, bool on,
+                                          int64_t ldb, int64_t safe, int g, Held &h) {
+  const int r = g + kRowLanes * J;
+  const bool ok = on && r < nb;
+  const int64_t idx = ok ? b_index<LEFT>(lo, r, rhs, ldb) : safe;
+  const T re = b[2 * idx], im = b[2 * idx + 1];
+  h.at<J>() = Cd{ok ? (double)re : 0.0, ok ? (double)im : 0.0};
+}
+
+template <typename T, bool LEFT, int J>
+__device__ __forceinline__ void leaf_store(T *b, int64_t lo, int nb, int64_t rhs, bool on,
+                                           int64_t ldb, int g, Held &h) {
+  const int r = g + kRowLanes * J;
+  if (on && r < nb) {
+    T *o = b + 2 * b_index<LEFT>(lo, r, rhs, ldb);
+    o[0] = (T)h.at<J>().re;
+    o[1] = (T)h.at<J>().im;
+  }
+}
+
+// Solves the diagonal block [lo, lo + nb) of op(A) for `nrhs` right-hand
+// sides. FORWARD: Left + effective lower, or Right + effective upper.
+template <typename T, bool LEFT, bool FORWARD>
+__global__ __launch_bounds__(kLeafThreads) void ctrsm_leaf(const T *a, int64_t lda, int trans,
+                                                           int unit, int64_t lo, int nb,
+                                                           int64_t nrhs, T *b, int64_t ldb) {
+  // U[i][r]: coefficient of x_i in the update of entry r (op(A)(r, i) for
+  // Left, op(A)(i, r) for Right), for r after i in substitution order;
+  // U[i][i] the diagonal; zero elsewhere.
+  __shared__ Cd U[kNB][kNB];
+  const int64_t diag = lo + lo * lda;  // an element every block reads
+#pragma unroll
+  for (int t = 0; t < kNB * kNB / kLeafThreads; ++t) {
+    const int e = threadIdx.x + kLeafThreads * t;
+    const int r = e % kNB, i = e / kNB;
+    const bool in = r < nb && i < nb && (r == i || (FORWARD ? r > i : r < i));
+    const int64_t row = LEFT ? lo + r : lo + i, col = LEFT ? lo + i : lo + r;
+    const int64_t idx = in ? (trans ? col + row * lda : row + col * lda) : diag;
+    const T re = a[2 * idx], im = a[2 * idx + 1];
+    const double imv = trans == 2 ? 0.0 - (double)im : (double)im;
+    U[i][r] = Cd{in ? (double)re : 0.0, in ? imv : 0.0};
+  }
+  const int g = threadIdx.x % kRowLanes;
+  const int64_t rhs = blockIdx.x * (int64_t)kRhsPerBlock + threadIdx.x / kRowLanes;
+  const bool on = rhs < nrhs;
+  const int64_t safe = LEFT ? lo : lo * ldb;  // B(lo, 0) for Left, B(0, lo) for Right
+  Held h;
+  leaf_load<T, LEFT, 0>(b, lo, nb, rhs, on, ldb, safe, g, h);
+  leaf_load<T, LEFT, 1>(b, lo, nb, rhs, on, ldb, safe, g, h);
+  leaf_load<T, LEFT, 2>(b, lo, nb, rhs, on, ldb, safe, g, h);
+  leaf_load<T, LEFT, 3>(b, lo, nb, rhs, on, ldb, safe, g, h);
   __syncthreads();
-  if (!on) return;
-  for (int s = 0; s < nb; ++s) {
-    const int i = upper ? nb - 1 - s : s;
-    Cd acc = X[i][t];
-    const int k0 = upper ? i + 1 : 0, k1 = upper ? nb : i;
-    for (int kk = k0; kk < k1; ++kk) {
-      const Cd p = cmul(As[i][kk], X[kk][t]);
-      acc = Cd{acc.re - p.re, acc.im - p.im};
-    }
-    X[i][t] = unit ? acc : cdiv(acc, As[i][i]);
-  }
-  for (int r = 0; r < nb; ++r) {
-    col[2 * r] = (T)X[r][t].re;
-    col[2 * r + 1] = (T)X[r][t].im;
-  }
-}
-
-// Right: columns [lo, lo + nb) of B, row i = blockIdx.x * 64 + thread.
-template <typename T>
-__global__ __launch_bounds__(kNB) void ctrsm_leaf_right(const T *a, int64_t lda, int trans,
-                                                        int upper, int unit, int64_t lo, int nb,
-                                                        int64_t m, T *b, int64_t ldb) {
-  __shared__ Cd As[kNB][kNB + 1];
-  __shared__ Cd X[kNB][kNB];  // X[column][thread]
-  load_block(a, lda, trans, upper, lo, nb, As);
-  const int t = threadIdx.x;
-  const int64_t i = blockIdx.x * (int64_t)kNB + t;
-  const bool on = i < m;
-  if (on)
-    for (int c = 0; c < nb; ++c) {
-      const T *x = b + 2 * (i + (lo + c) * ldb);
-      X[c][t] = Cd{(double)x[0], (double)x[1]};
-    }
-  __syncthreads();
-  if (!on) return;
-  for (int s = 0; s < nb; ++s) {
-    const int j = upper ? s : nb - 1 - s;
-    Cd acc = X[j][t];
-    const int k0 = upper ? 0 : j + 1, k1 = upper ? j : nb;
-    for (int kk = k0; kk < k1; ++kk) {
-      const Cd p = cmul(X[kk][t], As[kk][j]);
-      acc = Cd{acc.re - p.re, acc.im - p.im};
-    }
-    X[j][t] = unit ? acc : cdiv(acc, As[j][j]);
-  }
-  for (int c = 0; c < nb; ++c) {
-    T *x = b + 2 * (i + (lo + c) * ldb);
-    x[0] = (T)X[c][t].re;
-    x[1] = (T)X[c][t].im;
-  }
+  LeafRun<LEFT, FORWARD, 0>::go(U, nb, unit, g, h);
+  leaf_store<T, LEFT, 0>(b, lo, nb, rhs, on, ldb, g, h);
+  leaf_store<T, LEFT, 1>(b, lo, nb, rhs, on, ldb, g, h);
+  leaf_store<T, LEFT, 2>(b, lo, nb, rhs, on, ldb, g, h);
+  leaf_store<T, LEFT, 3>(b, lo, nb, rhs, on, ldb, g, h);
 }
 
 unsigned grid_for(int64_t total) {
@@ -156,18 +236,23 @@ struct CSolver {
   hipStream_t st;
   bool upper, forward;
 
+  template <bool LEFT, bool FORWARD>
+  hipError_t leaf_as(int64_t lo, int nb, int64_t nrhs) {
+    hipLaunchKernelGGL((ctrsm_leaf<T, LEFT, FORWARD>),
+                       dim3((unsigned)((nrhs + kRhsPerBlock - 1) / kRhsPerBlock)),
+                       dim3(kLeafThreads), 0, st, p.a, p.lda, p.trans, (int)p.unit, lo, nb, nrhs,
+                       p.b, p.ldb);
+    return hipGetLastError();
+  }
+
+  hipError_t leaf(int64_t lo, int nb) {
+    if (p.left)
+      return forward ? leaf_as<true, true>(lo, nb, p.n) : leaf_as<true, false>(lo, nb, p.n);
+    return forward ? leaf_as<false, true>(lo, nb, p.m) : leaf_as<false, false>(lo, nb, p.m);
+  }
+
   hipError_t range(int64_t lo, int64_t len) {
-    if (len <= kNB) {
-      if (p.left)
-        hipLaunchKernelGGL((ctrsm_leaf_left<T>), dim3((unsigned)((p.n + kNB - 1) / kNB)),
-                           dim3(kNB), 0, st, p.a, p.lda, p.trans, (int)upper, (int)p.unit, lo,
-                           (int)len, p.n, p.b, p.ldb);
-      else
-        hipLaunchKernelGGL((ctrsm_leaf_right<T>), dim3((unsigned)((p.m + kNB - 1) / kNB)),
-                           dim3(kNB), 0, st, p.a, p.lda, p.trans, (int)upper, (int)p.unit, lo,
-                           (int)len, p.m, p.b, p.ldb);
-      return hipGetLastError();
-    }
+    if (len <= kNB) return leaf(lo, (int)len);
     const int64_t h = ((len / 2 + kNB - 1) / kNB) * kNB;
     int64_t f_lo = lo, f_len = h, s_lo = lo + h, s_len = len - h;
     if (!forward) {
