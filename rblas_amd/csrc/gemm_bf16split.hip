@@ -345,20 +345,25 @@ __global__ __launch_bounds__(kThreads) void split_gemm_kernel(SplitArgs p) {
     __threadfence();
     // acc := slot 0 + slot 1 + ... in split order (this split's own slot is
     // read back too, so the sum is the same whichever split finishes last).
-    // Splits outermost: each pass issues all of its loads at once.
+    // One block row (64 values) per pass, its loads issued together: a whole
+    // slot at once (256 values beside the 256 accumulators) does not fit in
+    // registers and would spill.
     const float *base = p.partials + (int64_t)tile * p.splits * kSlot + threadIdx.x;
-    for (int q = 0; q < p.splits; ++q) {
-      const float *slot = base + (int64_t)q * kSlot;
 #pragma unroll
-      for (int i = 0; i < 4; ++i)
+    for (int i = 0; i < 4; ++i)
+      for (int q = 0; q < p.splits; ++q) {
+        const float *slot = base + (int64_t)q * kSlot;
+        float v[4][16];
 #pragma unroll
         for (int j = 0; j < 4; ++j)
 #pragma unroll
-          for (int r = 0; r < 16; ++r) {
-            const float v = slot[((i * 4 + j) * 16 + r) * kThreads];
-            g.acc[i][j][r] = q == 0 ? v : g.acc[i][j][r] + v;
-          }
-    }
+          for (int r = 0; r < 16; ++r) v[j][r] = slot[((i * 4 + j) * 16 + r) * kThreads];
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+#pragma unroll
+          for (int r = 0; r < 16; ++r)
+            g.acc[i][j][r] = q == 0 ? v[j][r] : g.acc[i][j][r] + v[j][r];
+      }
     if (threadIdx.x == 0) atomicExch(&p.counters[tile], 0);
   }
 
