@@ -12,6 +12,7 @@
 // rblas's complex multiply (complex.rs:101-111: re = fma(a.re, b.re,
 // -(a.im*b.im)), im = fma(a.re, b.im, a.im*b.re)).
 #include "kernels.hpp"
+#include "fma.hpp"
 
 namespace rblas {
 namespace {
@@ -73,11 +74,11 @@ __global__ void cplx_combine(int64_t m, int64_t n, T ar, T ai, const T *pr, cons
     T *cp = c + 2 * (i + j * ldc);
     if (!beta_zero) {
       const T c_re = cp[0], c_im = cp[1];
-      s_re = __builtin_fma(br, c_re, -(bi * c_im));
-      s_im = __builtin_fma(br, c_im, bi * c_re);
+      s_re = fma_t(br, c_re, -(bi * c_im));
+      s_im = fma_t(br, c_im, bi * c_re);
     }
-    const T q_re = __builtin_fma(ar, p_re, -(ai * p_im));
-    const T q_im = __builtin_fma(ar, p_im, ai * p_re);
+    const T q_re = fma_t(ar, p_re, -(ai * p_im));
+    const T q_im = fma_t(ar, p_im, ai * p_re);
     cp[0] = q_re + s_re;
     cp[1] = q_im + s_im;
   }

@@ -15,6 +15,7 @@
 #include <cmath>
 
 #include "kernels.hpp"
+#include "fma.hpp"
 
 namespace rblas {
 namespace {
@@ -51,8 +52,8 @@ __global__ void cscale_kernel(int64_t m, int64_t n, T ar, T ai, T *b, int64_t ld
     const int64_t j = e / m, i = e - j * m;
     T *x = b + 2 * (i + j * ldb);
     const T xr = x[0], xi = x[1];
-    x[0] = __builtin_fma(ar, xr, T(0) - ai * xi);
-    x[1] = __builtin_fma(ar, xi, ai * xr);
+    x[0] = fma_t(ar, xr, T(0) - ai * xi);
+    x[1] = fma_t(ar, xi, ai * xr);
   }
 }
 

@@ -14,6 +14,7 @@
 //       TOMS 44(1), 2017): squares of small, medium and large magnitudes are
 //       summed in separately scaled accumulators, which add across threads.
 #include "kernels.hpp"
+#include "fma.hpp"
 
 namespace rblas {
 namespace {
@@ -67,11 +68,11 @@ __global__ __launch_bounds__(kThreads) void dot_partial(int64_t n, const T *x, i
     for (int64_t i = tid; i < nv; i += stride) {
       const V a = xv[i], b = yv[i];
 #pragma unroll
-      for (int e = 0; e < W; ++e) (e & 1 ? acc1 : acc0) = __builtin_fma(a[e], b[e], e & 1 ? acc1 : acc0);
+      for (int e = 0; e < W; ++e) (e & 1 ? acc1 : acc0) = fma_t(a[e], b[e], e & 1 ? acc1 : acc0);
     }
-    for (int64_t i = nv * W + tid; i < n; i += stride) acc0 = __builtin_fma(x[i], y[i], acc0);
+    for (int64_t i = nv * W + tid; i < n; i += stride) acc0 = fma_t(x[i], y[i], acc0);
   } else {
-    for (int64_t i = tid; i < n; i += stride) acc0 = __builtin_fma(x[i * incx], y[i * incy], acc0);
+    for (int64_t i = tid; i < n; i += stride) acc0 = fma_t(x[i * incx], y[i * incy], acc0);
   }
   const T s = block_sum(acc0 + acc1, sh);
   if (threadIdx.x == 0) partial[blockIdx.x] = s;
@@ -102,13 +103,13 @@ __global__ __launch_bounds__(kThreads) void axpy_kernel(int64_t n, T alpha, cons
       const V a = xv[i];
       V b = yv[i];
 #pragma unroll
-      for (int e = 0; e < W; ++e) b[e] = __builtin_fma(alpha, a[e], b[e]);
+      for (int e = 0; e < W; ++e) b[e] = fma_t(alpha, a[e], b[e]);
       yv[i] = b;
     }
-    for (int64_t i = nv * W + tid; i < n; i += stride) y[i] = __builtin_fma(alpha, x[i], y[i]);
+    for (int64_t i = nv * W + tid; i < n; i += stride) y[i] = fma_t(alpha, x[i], y[i]);
   } else {
     for (int64_t i = tid; i < n; i += stride)
-      y[i * incy] = __builtin_fma(alpha, x[i * incx], y[i * incy]);
+      y[i * incy] = fma_t(alpha, x[i * incx], y[i * incy]);
   }
 }
 

@@ -37,6 +37,7 @@
 // hold 16 consecutive rows of one C column and the epilogue writes runs of
 // column-major C.
 #pragma once
+#include "fma.hpp"
 #include "kernels.hpp"
 
 namespace rblas {
@@ -491,7 +492,7 @@ struct TileGemm {
 #pragma unroll
           for (int r = 0; r < 4; ++r) {
             const T v = acc[i0 + ii][j][r];
-            const T out = read_c ? __builtin_fma(alpha, v, beta * cv[ii][j][r]) : alpha * v;
+            const T out = read_c ? fma_t(alpha, v, beta * cv[ii][j][r]) : alpha * v;
             buffer_store<T>(rs, elem_off(p, m0, n0, i0 + ii, j, r, lane_byte), out);
           }
     }
