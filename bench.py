@@ -30,7 +30,9 @@ synthetic data of the stated shape:
   - the small SGEMM (col-major NN 256^3, U[-1,1] seed 0, beta 0.5): device
     time per call and host-slice call time;
   - the headline DGEMM through the host-slice entry point (PCIe-inclusive);
-  - C64 gemm (4M) 4096^3 and Level-1 bandwidth (f64 and f32).
+  - complex trsm (ZTRSM / CTRSM 4096 x 4096, Left and Right);
+  - C64 gemm (4M) 4096^3, Level-1 bandwidth (f64 and f32) and Level-2
+    bandwidth (gemv NoTrans / Trans, ger; f64, f32, complex f64).
 `roofline` prices the DGEMM kernel against MI355X's FP64 MFMA peak, with HBM
 traffic per launch from the committed rocprofv3 PMC summary; `cpu_baseline`
 times the oracle (rblas's CPU algorithm restated in C++) on a column slice
@@ -442,6 +444,49 @@ def ctrsm_single():
     return out
 
 
+def level2_bw(n=16384, steps=10):
+    """gemv / ger on resident operands, column-major, ld q + 7: DGEMV and
+    SGEMV NoTrans and Trans at q = n, ZGEMV NoTrans and ConjTrans at q = n/2,
+    DGER at q = n (one N(0,1) buffer, seed 31, serves every case).
+    Algorithmic bytes: gemv reads A once plus x and y and writes y; ger reads
+    and writes A and reads x and y."""
+    from rblas_amd import level2, workloads
+    from rblas_amd.device import DeviceArray, Stream
+    s = Stream()
+    out = {}
+    base = workloads.normal(31, 0, n * (n + 7) + 4 * n, np.float64)
+    cases = (("dgemv_n", np.float64, n, 0), ("dgemv_t", np.float64, n, 1),
+             ("dger", np.float64, n, None),
+             ("sgemv_n", np.float32, n, 0), ("sgemv_t", np.float32, n, 1),
+             ("zgemv_n", np.complex128, n // 2, 0), ("zgemv_c", np.complex128, n // 2, 2))
+    for name, dt, q, trans in cases:
+        ld = q + 7
+        per = 2 if np.dtype(dt).kind == "c" else 1
+        la = per * ld * q
+        real = np.float32 if dt == np.float32 else np.float64
+        a = base[:la].astype(real).view(dt)
+        x = base[la:la + per * q].astype(real).view(dt)
+        y = base[la + per * q:la + 2 * per * q].astype(real).view(dt)
+        da, dx, dy = (DeviceArray.from_host(v, s) for v in (a, x, y))
+        del a
+        es = np.dtype(dt).itemsize
+        if trans is None:
+            ms = timed(lambda: level2.ger_dev(1, q, q, 0.5, dx, 1, dy, 1, da, ld, s), s, steps, 2)
+            nbytes = 2 * q * q * es + 2 * q * es
+        else:
+            al, be = (1.5, -0.25) if per == 1 else (1.5 - 0.5j, 0.25j)
+            ms = timed(lambda: level2.gemv_dev(1, trans, q, q, al, da, ld, dx, 1, be, dy, 1, s),
+                       s, steps, 2)
+            nbytes = q * q * es + 3 * q * es
+        gbs = nbytes / ms / 1e6
+        out[f"{name}_{q}"] = {"ms": round(ms, 4), "gb_s": round(gbs, 1),
+                              "hbm_frac": round(gbs / HBM_PEAK_GBS, 3)}
+        for v in (da, dx, dy):
+            v.free()
+    s.close()
+    return out
+
+
 def serving(count):
     from rblas_amd import serve
     closed = serve.workload(count, 0.0)
@@ -537,6 +582,7 @@ ONLY = {
     "trsm": lambda a: trsm_single(),
     "symm_hemm": lambda a: symm_hemm(),
     "ctrsm": lambda a: ctrsm_single(),
+    "level2": lambda a: level2_bw(),
     "serve": lambda a: serving(a.serve_requests),
     "c64_gemm": lambda a: c64_gemm(),
     "sgemm_skinny": lambda a: skinny_sgemm_tflops(),
@@ -630,6 +676,8 @@ def main():
         log("c64 / level-1 ...")
         comp["c64_gemm_4096"] = c64_gemm()
         comp["level1"] = level1_bw()
+        log("level-2 ...")
+        comp["level2"] = level2_bw()
         out["components"] = comp
     if not args.no_cpu_baseline:
         log("cpu baseline ...")

@@ -374,6 +374,104 @@ int32_t rblas_daxpy_dev(size_t n, double alpha, const double *x, size_t x_len,
                         size_t incx, double *y, size_t y_len, size_t incy,
                         void *stream);
 
+/* ----------------------------------------------------------------- level-2
+ * gemv: y := alpha op(A) x + beta y; ger: A := alpha x y^T + A. A is m x n
+ * in `layout` with leading dimension lda ((rs, cs) = RowMajor (lda, 1),
+ * ColMajor (1, lda), types.rs:20-25); x and y are strided by incx / incy.
+ * rblas checks nothing up front: RBLAS_PANIC where its loops would index past
+ * a slice (gemv: y over its length when beta != 1, real, or always, complex;
+ * then, when alpha != 0, a and x for every output with a non-empty sum and
+ * y; the real Trans loop reads x_i for every row. ger: nothing when
+ * alpha == 0, else x_i for every row, then y_j and a_ij when n > 0).
+ * RBLAS_BAD_ARG where the device path cannot follow rblas's sequential
+ * order: gemv with incy == 0 and more than one output, ger with lda smaller
+ * than the contiguous dimension (aliased elements), dimensions >= 2^31.
+ * Real (level2.rs:11-104): beta == 1 leaves y, beta == 0 stores 0, then
+ * y = fma(alpha, sum, y). Complex (complex2.rs:11-142): y = beta y always
+ * (0 when beta == 0), then y + alpha sum with rblas's complex multiply;
+ * ConjTrans conjugates A; gerc conjugates y. Sums run in a different order
+ * from rblas's sequential loops (lanes, then splits). Lengths and strides
+ * of complex operands count complex elements. */
+int32_t rblas_sgemv(int32_t layout, int32_t trans, size_t m, size_t n, float alpha,
+                    const float *a, size_t a_len, size_t lda, const float *x, size_t x_len,
+                    size_t incx, float beta, float *y, size_t y_len, size_t incy);
+int32_t rblas_dgemv(int32_t layout, int32_t trans, size_t m, size_t n, double alpha,
+                    const double *a, size_t a_len, size_t lda, const double *x, size_t x_len,
+                    size_t incx, double beta, double *y, size_t y_len, size_t incy);
+int32_t rblas_cgemv(int32_t layout, int32_t trans, size_t m, size_t n, rblas_complex32_t alpha,
+                    const rblas_complex32_t *a, size_t a_len, size_t lda,
+                    const rblas_complex32_t *x, size_t x_len, size_t incx, rblas_complex32_t beta,
+                    rblas_complex32_t *y, size_t y_len, size_t incy);
+int32_t rblas_zgemv(int32_t layout, int32_t trans, size_t m, size_t n, rblas_complex64_t alpha,
+                    const rblas_complex64_t *a, size_t a_len, size_t lda,
+                    const rblas_complex64_t *x, size_t x_len, size_t incx, rblas_complex64_t beta,
+                    rblas_complex64_t *y, size_t y_len, size_t incy);
+/* level2::ger (level2.rs:65-88); complex2::geru / gerc (complex2.rs:80-142). */
+int32_t rblas_sger(int32_t layout, size_t m, size_t n, float alpha, const float *x, size_t x_len,
+                   size_t incx, const float *y, size_t y_len, size_t incy, float *a, size_t a_len,
+                   size_t lda);
+int32_t rblas_dger(int32_t layout, size_t m, size_t n, double alpha, const double *x,
+                   size_t x_len, size_t incx, const double *y, size_t y_len, size_t incy,
+                   double *a, size_t a_len, size_t lda);
+int32_t rblas_cgeru(int32_t layout, size_t m, size_t n, rblas_complex32_t alpha,
+                    const rblas_complex32_t *x, size_t x_len, size_t incx,
+                    const rblas_complex32_t *y, size_t y_len, size_t incy, rblas_complex32_t *a,
+                    size_t a_len, size_t lda);
+int32_t rblas_zgeru(int32_t layout, size_t m, size_t n, rblas_complex64_t alpha,
+                    const rblas_complex64_t *x, size_t x_len, size_t incx,
+                    const rblas_complex64_t *y, size_t y_len, size_t incy, rblas_complex64_t *a,
+                    size_t a_len, size_t lda);
+int32_t rblas_cgerc(int32_t layout, size_t m, size_t n, rblas_complex32_t alpha,
+                    const rblas_complex32_t *x, size_t x_len, size_t incx,
+                    const rblas_complex32_t *y, size_t y_len, size_t incy, rblas_complex32_t *a,
+                    size_t a_len, size_t lda);
+int32_t rblas_zgerc(int32_t layout, size_t m, size_t n, rblas_complex64_t alpha,
+                    const rblas_complex64_t *x, size_t x_len, size_t incx,
+                    const rblas_complex64_t *y, size_t y_len, size_t incy, rblas_complex64_t *a,
+                    size_t a_len, size_t lda);
+/* HBM-resident forms (no rblas counterpart): the same checks, all pointers
+ * device pointers, enqueued on `stream`. */
+int32_t rblas_sgemv_dev(int32_t layout, int32_t trans, size_t m, size_t n, float alpha,
+                        const float *a, size_t a_len, size_t lda, const float *x, size_t x_len,
+                        size_t incx, float beta, float *y, size_t y_len, size_t incy,
+                        void *stream);
+int32_t rblas_dgemv_dev(int32_t layout, int32_t trans, size_t m, size_t n, double alpha,
+                        const double *a, size_t a_len, size_t lda, const double *x, size_t x_len,
+                        size_t incx, double beta, double *y, size_t y_len, size_t incy,
+                        void *stream);
+int32_t rblas_cgemv_dev(int32_t layout, int32_t trans, size_t m, size_t n,
+                        rblas_complex32_t alpha, const rblas_complex32_t *a, size_t a_len,
+                        size_t lda, const rblas_complex32_t *x, size_t x_len, size_t incx,
+                        rblas_complex32_t beta, rblas_complex32_t *y, size_t y_len, size_t incy,
+                        void *stream);
+int32_t rblas_zgemv_dev(int32_t layout, int32_t trans, size_t m, size_t n,
+                        rblas_complex64_t alpha, const rblas_complex64_t *a, size_t a_len,
+                        size_t lda, const rblas_complex64_t *x, size_t x_len, size_t incx,
+                        rblas_complex64_t beta, rblas_complex64_t *y, size_t y_len, size_t incy,
+                        void *stream);
+int32_t rblas_sger_dev(int32_t layout, size_t m, size_t n, float alpha, const float *x,
+                       size_t x_len, size_t incx, const float *y, size_t y_len, size_t incy,
+                       float *a, size_t a_len, size_t lda, void *stream);
+int32_t rblas_dger_dev(int32_t layout, size_t m, size_t n, double alpha, const double *x,
+                       size_t x_len, size_t incx, const double *y, size_t y_len, size_t incy,
+                       double *a, size_t a_len, size_t lda, void *stream);
+int32_t rblas_cgeru_dev(int32_t layout, size_t m, size_t n, rblas_complex32_t alpha,
+                        const rblas_complex32_t *x, size_t x_len, size_t incx,
+                        const rblas_complex32_t *y, size_t y_len, size_t incy,
+                        rblas_complex32_t *a, size_t a_len, size_t lda, void *stream);
+int32_t rblas_zgeru_dev(int32_t layout, size_t m, size_t n, rblas_complex64_t alpha,
+                        const rblas_complex64_t *x, size_t x_len, size_t incx,
+                        const rblas_complex64_t *y, size_t y_len, size_t incy,
+                        rblas_complex64_t *a, size_t a_len, size_t lda, void *stream);
+int32_t rblas_cgerc_dev(int32_t layout, size_t m, size_t n, rblas_complex32_t alpha,
+                        const rblas_complex32_t *x, size_t x_len, size_t incx,
+                        const rblas_complex32_t *y, size_t y_len, size_t incy,
+                        rblas_complex32_t *a, size_t a_len, size_t lda, void *stream);
+int32_t rblas_zgerc_dev(int32_t layout, size_t m, size_t n, rblas_complex64_t alpha,
+                        const rblas_complex64_t *x, size_t x_len, size_t incx,
+                        const rblas_complex64_t *y, size_t y_len, size_t incy,
+                        rblas_complex64_t *a, size_t a_len, size_t lda, void *stream);
+
 /* ------------------------------------------------------------ grouped gemm
  * No rblas counterpart: rblas runs one `level3::gemm` (level3.rs:32-62) per
  * call. A grouped plan holds many independent gemm problems, each with

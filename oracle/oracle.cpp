@@ -22,6 +22,10 @@
 //   complex gemm (4M)   : complex3.rs:373-437 cases + numpy complex matmul.
 //   dot / axpy / nrm2   : level1.rs:170-228 KATs (32, [12,24,36,48], 5,
 //                         sqrt(3)*1e200), numpy.
+//   complex trsm        : complex3.rs:439-478 round trip + numpy solve.
+//   gemv / ger          : level2.rs:110-187 KATs, numpy (all layouts,
+//                         transposes, strides), exact small integers.
+//   complex gemv / geru / gerc : complex2.rs:150-182 KATs, numpy.
 #include <immintrin.h>
 
 #include <algorithm>
@@ -475,6 +479,123 @@ void ctrsm(int side, int uplo, int trans, int diag, size_t m, size_t n, T alr, T
   }
 }
 
+// level2.rs:11-61 — gemv. (rs, cs) = layout.strides(lda): RowMajor (lda, 1),
+// ColMajor (1, lda). y := beta y unless beta == 1 (0 stores 0); nothing more
+// when alpha == 0; NoTrans: per row an fma chain over j, then
+// y_i = fma(alpha, acc, y_i); Trans: xi = alpha * x_i, then
+// y_j = fma(a_ij, xi, y_j) for i outer, j inner.
+template <typename T>
+void gemv(int layout, int trans, size_t m, size_t n, T alpha, const T *a, size_t lda, const T *x,
+          size_t incx, T beta, T *y, size_t incy) {
+  const size_t rs = layout == 0 ? lda : 1, cs = layout == 0 ? 1 : lda;
+  const bool tr = trans != 0;
+  const size_t leny = tr ? n : m;
+  if (beta != T(1))  // level2.rs:91-104
+    for (size_t t = 0; t < leny; ++t) y[t * incy] = beta == T(0) ? T(0) : beta * y[t * incy];
+  if (alpha == T(0)) return;
+  if (!tr) {
+    for (size_t i = 0; i < m; ++i) {
+      T acc = T(0);
+      for (size_t j = 0; j < n; ++j) acc = std::fma(a[i * rs + j * cs], x[j * incx], acc);
+      y[i * incy] = std::fma(alpha, acc, y[i * incy]);
+    }
+  } else {
+    for (size_t i = 0; i < m; ++i) {
+      const T xi = alpha * x[i * incx];
+      for (size_t j = 0; j < n; ++j) y[j * incy] = std::fma(a[i * rs + j * cs], xi, y[j * incy]);
+    }
+  }
+}
+
+// level2.rs:65-88 — ger: nothing when alpha == 0; a_ij = fma(alpha * x_i,
+// y_j, a_ij), i outer, j inner.
+template <typename T>
+void ger(int layout, size_t m, size_t n, T alpha, const T *x, size_t incx, const T *y,
+         size_t incy, T *a, size_t lda) {
+  if (alpha == T(0)) return;
+  const size_t rs = layout == 0 ? lda : 1, cs = layout == 0 ? 1 : lda;
+  for (size_t i = 0; i < m; ++i) {
+    const T axi = alpha * x[i * incx];
+    for (size_t j = 0; j < n; ++j) a[i * rs + j * cs] = std::fma(axi, y[j * incy], a[i * rs + j * cs]);
+  }
+}
+
+// complex2.rs:11-76 — complex gemv. y := beta y always (its skip condition
+// never holds; beta == 0 stores 0); nothing more when alpha == 0; per output
+// acc = sum of aval * x (complex multiply, then add), y = y + alpha * acc.
+// NoTrans: rows i over j; Trans / ConjTrans: columns j over i, ConjTrans
+// conjugating A.
+template <typename T>
+void cgemv(int layout, int trans, size_t m, size_t n, T alr, T ali, const T *a, size_t lda,
+           const T *x, size_t incx, T ber, T bei, T *y, size_t incy) {
+  const size_t rs = layout == 0 ? lda : 1, cs = layout == 0 ? 1 : lda;
+  const bool tr = trans != 0, conj = trans == 2;
+  const size_t leny = tr ? n : m;
+  const bool beta_zero = ber == T(0) && bei == T(0);
+  for (size_t t = 0; t < leny; ++t) {
+    T *yy = y + 2 * t * incy;
+    if (beta_zero) {
+      yy[0] = T(0);
+      yy[1] = T(0);
+    } else {
+      T re, im;
+      cmul(ber, bei, yy[0], yy[1], re, im);
+      yy[0] = re;
+      yy[1] = im;
+    }
+  }
+  if (alr == T(0) && ali == T(0)) return;
+  auto term = [&](size_t i, size_t j, size_t xi, T &sr, T &si) {
+    const T *av = a + 2 * (i * rs + j * cs);
+    const T ar = av[0], ai = conj ? T(0) - av[1] : av[1];
+    T re, im;
+    cmul(ar, ai, x[2 * xi], x[2 * xi + 1], re, im);
+    sr = sr + re;
+    si = si + im;
+  };
+  auto finish = [&](size_t o, T sr, T si) {
+    T *yy = y + 2 * o * incy;
+    T re, im;
+    cmul(alr, ali, sr, si, re, im);
+    yy[0] = yy[0] + re;
+    yy[1] = yy[1] + im;
+  };
+  if (!tr) {
+    for (size_t i = 0; i < m; ++i) {
+      T sr = T(0), si = T(0);
+      for (size_t j = 0; j < n; ++j) term(i, j, j * incx, sr, si);
+      finish(i, sr, si);
+    }
+  } else {
+    for (size_t j = 0; j < n; ++j) {
+      T sr = T(0), si = T(0);
+      for (size_t i = 0; i < m; ++i) term(i, j, i * incx, sr, si);
+      finish(j, sr, si);
+    }
+  }
+}
+
+// complex2.rs:112-142 — geru / gerc: nothing when alpha == 0;
+// axi = alpha * x_i, a_ij = a_ij + axi * y_j (y_j conjugated for gerc).
+template <typename T>
+void crank1(int layout, size_t m, size_t n, T alr, T ali, const T *x, size_t incx, const T *y,
+            size_t incy, T *a, size_t lda, bool conj) {
+  if (alr == T(0) && ali == T(0)) return;
+  const size_t rs = layout == 0 ? lda : 1, cs = layout == 0 ? 1 : lda;
+  for (size_t i = 0; i < m; ++i) {
+    T xr, xi;
+    cmul(alr, ali, x[2 * i * incx], x[2 * i * incx + 1], xr, xi);
+    for (size_t j = 0; j < n; ++j) {
+      const T yr = y[2 * j * incy], yi = conj ? T(0) - y[2 * j * incy + 1] : y[2 * j * incy + 1];
+      T re, im;
+      cmul(xr, xi, yr, yi, re, im);
+      T *av = a + 2 * (i * rs + j * cs);
+      av[0] = av[0] + re;
+      av[1] = av[1] + im;
+    }
+  }
+}
+
 // complex3.rs:27-125 — 4M complex gemm on split planes (column-major).
 // Interleaved {re, im} storage; trans 2 = ConjTrans conjugates.
 template <typename T>
@@ -652,6 +773,40 @@ void oracle_zgemm(int ta, int tb, size_t m, size_t n, size_t k, double alr, doub
                   const double *a, size_t lda, const double *b, size_t ldb, double ber,
                   double bei, double *c, size_t ldc, int threads) {
   cgemm<double>(ta, tb, m, n, k, alr, ali, a, lda, b, ldb, ber, bei, c, ldc, threads);
+}
+void oracle_sgemv(int layout, int trans, size_t m, size_t n, float alpha, const float *a,
+                  size_t lda, const float *x, size_t incx, float beta, float *y, size_t incy) {
+  gemv<float>(layout, trans, m, n, alpha, a, lda, x, incx, beta, y, incy);
+}
+void oracle_dgemv(int layout, int trans, size_t m, size_t n, double alpha, const double *a,
+                  size_t lda, const double *x, size_t incx, double beta, double *y, size_t incy) {
+  gemv<double>(layout, trans, m, n, alpha, a, lda, x, incx, beta, y, incy);
+}
+void oracle_sger(int layout, size_t m, size_t n, float alpha, const float *x, size_t incx,
+                 const float *y, size_t incy, float *a, size_t lda) {
+  ger<float>(layout, m, n, alpha, x, incx, y, incy, a, lda);
+}
+void oracle_dger(int layout, size_t m, size_t n, double alpha, const double *x, size_t incx,
+                 const double *y, size_t incy, double *a, size_t lda) {
+  ger<double>(layout, m, n, alpha, x, incx, y, incy, a, lda);
+}
+void oracle_cgemv(int layout, int trans, size_t m, size_t n, float alr, float ali, const float *a,
+                  size_t lda, const float *x, size_t incx, float ber, float bei, float *y,
+                  size_t incy) {
+  cgemv<float>(layout, trans, m, n, alr, ali, a, lda, x, incx, ber, bei, y, incy);
+}
+void oracle_zgemv(int layout, int trans, size_t m, size_t n, double alr, double ali,
+                  const double *a, size_t lda, const double *x, size_t incx, double ber,
+                  double bei, double *y, size_t incy) {
+  cgemv<double>(layout, trans, m, n, alr, ali, a, lda, x, incx, ber, bei, y, incy);
+}
+void oracle_crank1(int layout, size_t m, size_t n, float alr, float ali, const float *x,
+                   size_t incx, const float *y, size_t incy, float *a, size_t lda, int conj) {
+  crank1<float>(layout, m, n, alr, ali, x, incx, y, incy, a, lda, conj != 0);
+}
+void oracle_zrank1(int layout, size_t m, size_t n, double alr, double ali, const double *x,
+                   size_t incx, const double *y, size_t incy, double *a, size_t lda, int conj) {
+  crank1<double>(layout, m, n, alr, ali, x, incx, y, incy, a, lda, conj != 0);
 }
 void oracle_ctrsm(int side, int uplo, int trans, int diag, size_t m, size_t n, float alr,
                   float ali, const float *a, size_t lda, float *b, size_t ldb) {

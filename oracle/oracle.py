@@ -9,6 +9,8 @@ algorithms, see oracle.cpp) plus restatements of rblas's argument checks:
   syrk_panics / trsm_panics       level3.rs:236-294 / 390-520 (index walks)
   symm / hesy and their panics    level3.rs:301-381, complex3.rs:127-247
   ctrsm (C++)                     complex3.rs:252-349 (panics: trsm_panics, same loops)
+  gemv / ger / geru / gerc (C++)  level2.rs:11-104, complex2.rs:11-142
+  gemv_panics / ger_panics        their index walks
 
 All arrays are numpy, 1-D "slices" exactly as rblas takes them.
 """
@@ -73,6 +75,20 @@ def lib() -> C.CDLL:
             fn = getattr(_lib, f"oracle_{pre}nrm2")
             fn.argtypes = [sz, tp, sz]
             fn.restype = t
+        for pre, t, tp in (("s", C.c_float, f32p), ("d", C.c_double, f64p)):
+            fn = getattr(_lib, f"oracle_{pre}gemv")
+            fn.argtypes = [i, i, sz, sz, t, tp, sz, tp, sz, t, tp, sz]
+            fn.restype = None
+            fn = getattr(_lib, f"oracle_{pre}ger")
+            fn.argtypes = [i, sz, sz, t, tp, sz, tp, sz, tp, sz]
+            fn.restype = None
+        for pre, t, tp in (("c", C.c_float, f32p), ("z", C.c_double, f64p)):
+            fn = getattr(_lib, f"oracle_{pre}gemv")
+            fn.argtypes = [i, i, sz, sz, t, t, tp, sz, tp, sz, t, t, tp, sz]
+            fn.restype = None
+            fn = getattr(_lib, f"oracle_{pre}rank1")
+            fn.argtypes = [i, sz, sz, t, t, tp, sz, tp, sz, tp, sz, i]
+            fn.restype = None
         _lib.oracle_has_avx2.restype = i
     return _lib
 
@@ -137,6 +153,52 @@ def axpy(n, alpha, x, incx, y, incy):
 
 def nrm2(n, x, incx):
     return getattr(lib(), f"oracle_{_pre(x)}nrm2")(n, _ptr(x), incx)
+
+
+def _cpre(x: np.ndarray) -> tuple[str, type]:
+    if x.dtype == np.complex64:
+        return "c", np.float32
+    if x.dtype == np.complex128:
+        return "z", np.float64
+    raise TypeError(f"oracle: unsupported dtype {x.dtype}")
+
+
+def gemv(layout, trans, m, n, alpha, a, lda, x, incx, beta, y, incy):
+    """level2::gemv (level2.rs:11-61) or complex2::gemv (complex2.rs:11-76)
+    by y's dtype; in place on y."""
+    if np.iscomplexobj(y):
+        pre, real = _cpre(y)
+        al, be = complex(alpha), complex(beta)
+        getattr(lib(), f"oracle_{pre}gemv")(layout, trans, m, n, al.real, al.imag,
+                                            _ptr(a.view(real)), lda, _ptr(x.view(real)), incx,
+                                            be.real, be.imag, _ptr(y.view(real)), incy)
+        return
+    getattr(lib(), f"oracle_{_pre(y)}gemv")(layout, trans, m, n, alpha, _ptr(a), lda, _ptr(x),
+                                            incx, beta, _ptr(y), incy)
+
+
+def ger(layout, m, n, alpha, x, incx, y, incy, a, lda):
+    """level2::ger (level2.rs:65-88); in place on a."""
+    getattr(lib(), f"oracle_{_pre(a)}ger")(layout, m, n, alpha, _ptr(x), incx, _ptr(y), incy,
+                                           _ptr(a), lda)
+
+
+def _rank1(layout, m, n, alpha, x, incx, y, incy, a, lda, conj):
+    pre, real = _cpre(a)
+    al = complex(alpha)
+    getattr(lib(), f"oracle_{pre}rank1")(layout, m, n, al.real, al.imag, _ptr(x.view(real)), incx,
+                                         _ptr(y.view(real)), incy, _ptr(a.view(real)), lda,
+                                         int(conj))
+
+
+def geru(layout, m, n, alpha, x, incx, y, incy, a, lda):
+    """complex2::geru (complex2.rs:80-93); in place on a."""
+    _rank1(layout, m, n, alpha, x, incx, y, incy, a, lda, False)
+
+
+def gerc(layout, m, n, alpha, x, incx, y, incy, a, lda):
+    """complex2::gerc (complex2.rs:97-110); in place on a."""
+    _rank1(layout, m, n, alpha, x, incx, y, incy, a, lda, True)
 
 
 def has_avx2() -> bool:
@@ -391,5 +453,60 @@ def hesy_panics(side, uplo, m, n, a_len, lda, b_len, ldb, c_len, ldc):
     for j in range(n):
         for i in range(m):
             if i + j * ldc >= c_len:
+                return True
+    return False
+
+
+def gemv_panics(layout, trans, m, n, alpha, beta, a_len, lda, x_len, incx, y_len, incy,
+                complex_=False):
+    """level2.rs:11-104 / complex2.rs:11-76 walked literally (small sizes):
+    every a / x / y index the loops touch, in order. True if any reaches past
+    its slice. Real: y scaled only when beta != 1; complex: always."""
+    rs, cs = (lda, 1) if layout == ROW_MAJOR else (1, lda)
+    tr = trans != NO_TRANS
+    leny = n if tr else m
+    bad = [False]
+
+    def touch(i, length):
+        if i >= length:
+            bad[0] = True
+
+    if complex_ or beta != 1:
+        for t in range(leny):
+            touch(t * incy, y_len)
+    if alpha == 0:
+        return bad[0]
+    if not tr:
+        for i in range(m):
+            for j in range(n):
+                touch(i * rs + j * cs, a_len)
+                touch(j * incx, x_len)
+            touch(i * incy, y_len)
+    elif complex_:
+        for j in range(n):
+            for i in range(m):
+                touch(i * rs + j * cs, a_len)
+                touch(i * incx, x_len)
+            touch(j * incy, y_len)
+    else:
+        for i in range(m):
+            touch(i * incx, x_len)
+            for j in range(n):
+                touch(i * rs + j * cs, a_len)
+                touch(j * incy, y_len)
+    return bad[0]
+
+
+def ger_panics(layout, m, n, alpha, x_len, incx, y_len, incy, a_len, lda):
+    """level2.rs:65-88 / complex2.rs:112-142 walked literally (small sizes):
+    x_i for every row (also when n == 0), then y_j and a_ij."""
+    if alpha == 0:
+        return False
+    rs, cs = (lda, 1) if layout == ROW_MAJOR else (1, lda)
+    for i in range(m):
+        if i * incx >= x_len:
+            return True
+        for j in range(n):
+            if j * incy >= y_len or i * rs + j * cs >= a_len:
                 return True
     return False

@@ -88,6 +88,14 @@ def _gemm_args(t):
     return [i32, i32, i32, sz, sz, sz, t, vp, sz, sz, vp, sz, sz, t, vp, sz, sz]
 
 
+def _gemv_args(t):
+    return [i32, i32, sz, sz, t, vp, sz, sz, vp, sz, sz, t, vp, sz, sz]
+
+
+def _ger_args(t):
+    return [i32, sz, sz, t, vp, sz, sz, vp, sz, sz, vp, sz, sz]
+
+
 # name -> (argtypes, restype). Every symbol here is declared in the header;
 # tests check that the library exports each of them.
 SIGNATURES: dict[str, tuple[list, object]] = {
@@ -140,6 +148,26 @@ SIGNATURES: dict[str, tuple[list, object]] = {
     "rblas_ztrsm": ([i32, i32, i32, i32, sz, sz, Complex64, vp, sz, sz, vp, sz, sz], i32),
     "rblas_ctrsm_dev": ([i32, i32, i32, i32, sz, sz, Complex32, vp, sz, sz, vp, sz, sz, vp], i32),
     "rblas_ztrsm_dev": ([i32, i32, i32, i32, sz, sz, Complex64, vp, sz, sz, vp, sz, sz, vp], i32),
+    "rblas_sgemv": (_gemv_args(f32), i32),
+    "rblas_dgemv": (_gemv_args(f64), i32),
+    "rblas_cgemv": (_gemv_args(Complex32), i32),
+    "rblas_zgemv": (_gemv_args(Complex64), i32),
+    "rblas_sgemv_dev": (_gemv_args(f32) + [vp], i32),
+    "rblas_dgemv_dev": (_gemv_args(f64) + [vp], i32),
+    "rblas_cgemv_dev": (_gemv_args(Complex32) + [vp], i32),
+    "rblas_zgemv_dev": (_gemv_args(Complex64) + [vp], i32),
+    "rblas_sger": (_ger_args(f32), i32),
+    "rblas_dger": (_ger_args(f64), i32),
+    "rblas_cgeru": (_ger_args(Complex32), i32),
+    "rblas_zgeru": (_ger_args(Complex64), i32),
+    "rblas_cgerc": (_ger_args(Complex32), i32),
+    "rblas_zgerc": (_ger_args(Complex64), i32),
+    "rblas_sger_dev": (_ger_args(f32) + [vp], i32),
+    "rblas_dger_dev": (_ger_args(f64) + [vp], i32),
+    "rblas_cgeru_dev": (_ger_args(Complex32) + [vp], i32),
+    "rblas_zgeru_dev": (_ger_args(Complex64) + [vp], i32),
+    "rblas_cgerc_dev": (_ger_args(Complex32) + [vp], i32),
+    "rblas_zgerc_dev": (_ger_args(Complex64) + [vp], i32),
     "rblas_sdot": ([sz, vp, sz, sz, vp, sz, sz, C.POINTER(f32)], i32),
     "rblas_ddot": ([sz, vp, sz, sz, vp, sz, sz, C.POINTER(f64)], i32),
     "rblas_snrm2": ([sz, vp, sz, sz, C.POINTER(f32)], i32),

@@ -93,6 +93,46 @@ size_t cgemm_workspace_elems(int64_t m, int64_t n, int64_t k);
 template <typename T>
 hipError_t launch_cgemm(const CgemmProblem<T> &p, T *ws, hipStream_t stream);
 
+// Level-2 over a column-major storage view S of A: R x C, S(r, c) =
+// a[r + c*ld] (complex: interleaved {re, im}; a RowMajor A is the view of its
+// transpose). gemv: cols_shape ? y_c = f(sum_r S(r,c) v_r) : y_r =
+// f(sum_c S(r,c) v_c), S conjugated when `conj`, f applying alpha and beta
+// (real: fma(alpha, acc, beta y) with beta 1 / 0 special; complex: beta y +
+// alpha acc). `ws` holds gemv_workspace_bytes(p) bytes (0: none needed).
+template <typename T>
+struct GemvProblem {
+  bool cols_shape, cplx, conj;
+  int64_t R, C;
+  const T *a;
+  int64_t ld;
+  const T *v;
+  int64_t incv;
+  T *y;
+  int64_t incy;
+  T alpha_re, alpha_im, beta_re, beta_im;
+};
+template <typename T>
+size_t gemv_workspace_bytes(const GemvProblem<T> &p);
+template <typename T>
+hipError_t launch_gemv(const GemvProblem<T> &p, void *ws, hipStream_t stream);
+
+// Rank-1 update of the storage view: logical (i, j) = (r, c), or (c, r) when
+// `rowmajor`; x is indexed by i, y by j (conjugated when `conj`).
+template <typename T>
+struct Rank1Problem {
+  bool cplx, conj, rowmajor;
+  int64_t R, C;
+  T *a;
+  int64_t ld;
+  const T *x;
+  int64_t incx;
+  const T *y;
+  int64_t incy;
+  T alpha_re, alpha_im;
+};
+template <typename T>
+hipError_t launch_rank1(const Rank1Problem<T> &p, hipStream_t stream);
+
 // Complex trsm (interleaved complex B and A): op(A) X = alpha B (left) or
 // X op(A) = alpha B, in place; `upper` is the stored triangle (uplo Upper),
 // trans 0 / 1 / 2 = None / Trans / ConjTrans.
