@@ -210,7 +210,17 @@ __global__ __launch_bounds__(kThreads) void gemv_finish(int64_t leny, int nsplit
   for (int64_t o = blockIdx.x * (int64_t)kThreads + threadIdx.x; o < leny;
        o += (int64_t)gridDim.x * kThreads) {
     Cx<T> acc = nsplit > 0 ? ld_elem<T, CPLX>(part, o) : Cx<T>{T(0), T(0)};
-    for (int q = 1; q < nsplit; ++q) {
+    // Eight partials in flight per round trip, added in split order.
+    constexpr int kBatch = 8;
+    int q = 1;
+    for (; q + kBatch <= nsplit; q += kBatch) {
+      Cx<T> pv[kBatch];
+#pragma unroll
+      for (int u = 0; u < kBatch; ++u) pv[u] = ld_elem<T, CPLX>(part, (q + u) * leny + o);
+#pragma unroll
+      for (int u = 0; u < kBatch; ++u) acc = Cx<T>{acc.re + pv[u].re, acc.im + pv[u].im};
+    }
+    for (; q < nsplit; ++q) {
       const Cx<T> p = ld_elem<T, CPLX>(part, q * leny + o);
       acc = Cx<T>{acc.re + p.re, acc.im + p.im};
     }
