@@ -7,11 +7,13 @@
 //      comes first in substitution order, subtract its contribution from the
 //      other half with one 4M complex gemm (alpha -1, beta 1), solve the other
 //      half.
-//   3. A range of up to 64 is solved by substitution with rblas's arithmetic
-//      (s = s - a*x, then cdiv(s, a_ii) = s conj(a_ii) / norm_sqr(a_ii)), in
-//      f64, 16 lanes per right-hand side (a column of B for Left, a row for
-//      Right) and the block of op(A) in LDS. A non-finite B entry reaches
-//      exactly the entries rblas's substitution reaches.
+//   3. A range of up to 64 is solved by substitution in f64, 16 lanes per
+//      right-hand side (a column of B for Left, a row for Right), the block
+//      of op(A) in LDS: s = s - a*x with rblas's complex multiply, then
+//      s * (1 / a_ii) with 1 / a_ii = conj(a_ii) / norm_sqr(a_ii) formed once
+//      per block where rblas divides each s by norm_sqr(a_ii)
+//      (complex3.rs:344-349): the same quotient up to rounding. A non-finite
+//      B entry reaches exactly the entries rblas's substitution reaches.
 #include <cmath>
 
 #include "kernels.hpp"
@@ -35,13 +37,6 @@ __device__ __forceinline__ Cd cmul(Cd a, Cd b) {
 // between their addresses and force both into memory.
 __device__ __forceinline__ Cd csel(bool c, Cd a, Cd b) {
   return Cd{c ? a.re : b.re, c ? a.im : b.im};
-}
-
-// complex3.rs:344-349: s * conj(d) / (fma(d.re, d.re, d.im * d.im)).
-__device__ __forceinline__ Cd cdiv(Cd s, Cd d) {
-  const double n = __builtin_fma(d.re, d.re, d.im * d.im);
-  const Cd num = cmul(s, Cd{d.re, 0.0 - d.im});
-  return Cd{num.re / n, num.im / n};
 }
 
 template <typename T>
@@ -100,14 +95,53 @@ struct Held {
 };
 static_assert(kPer == 4, "Held keeps four entries per lane");
 
+// The coefficients step I needs, loaded one step ahead: U[i][g + 16 J]
+// for the entries the step updates, and 1 / op(A)(i, i).
+struct StepCoef {
+  Cd c0, c1, c2, c3, dinv;
+  template <int J>
+  __device__ __forceinline__ const Cd &at() const {
+    if constexpr (J == 0) return c0;
+    else if constexpr (J == 1) return c1;
+    else if constexpr (J == 2) return c2;
+    else return c3;
+  }
+  template <int J>
+  __device__ __forceinline__ Cd &at() {
+    if constexpr (J == 0) return c0;
+    else if constexpr (J == 1) return c1;
+    else if constexpr (J == 2) return c2;
+    else return c3;
+  }
+};
+
+template <bool FORWARD, int I, int J>
+__device__ __forceinline__ void load_coef_j(const Cd (*U)[kNB], int g, StepCoef &k) {
+  constexpr int i = FORWARD ? I : kNB - 1 - I;
+  constexpr int jo = i / kRowLanes;
+  if constexpr (FORWARD ? J >= jo : J <= jo) k.at<J>() = U[i][g + kRowLanes * J];
+}
+
+template <bool FORWARD, bool UNIT, int I>
+__device__ __forceinline__ void load_coef(const Cd (*U)[kNB], const Cd *dinv, int g,
+                                          StepCoef &k) {
+  if constexpr (I < kNB) {
+    load_coef_j<FORWARD, I, 0>(U, g, k);
+    load_coef_j<FORWARD, I, 1>(U, g, k);
+    load_coef_j<FORWARD, I, 2>(U, g, k);
+    load_coef_j<FORWARD, I, 3>(U, g, k);
+    if constexpr (!UNIT) k.dinv = dinv[FORWARD ? I : kNB - 1 - I];
+  }
+}
+
 // Entry r = g + 16 J after x_i is known.
 template <bool LEFT, bool FORWARD, int I, int J>
-__device__ __forceinline__ void leaf_update(const Cd (*U)[kNB], int g, Cd xi, Held &h) {
+__device__ __forceinline__ void leaf_update(const StepCoef &k, int g, Cd xi, Held &h) {
   constexpr int i = FORWARD ? I : kNB - 1 - I;
   constexpr int jo = i / kRowLanes;
   if constexpr (FORWARD ? J >= jo : J <= jo) {
     const int r = g + kRowLanes * J;
-    const Cd c = U[i][r];
+    const Cd c = k.at<J>();
     const Cd p = LEFT ? cmul(c, xi) : cmul(xi, c);
     Cd &e = h.at<J>();
     const Cd upd{e.re - p.re, e.im - p.im};
@@ -121,34 +155,41 @@ __device__ __forceinline__ void leaf_update(const Cd (*U)[kNB], int g, Cd xi, He
 }
 
 // Step I of the solve in substitution order: entry i = FORWARD ? I : 63 - I.
-template <bool LEFT, bool FORWARD, int I>
-__device__ __forceinline__ void leaf_step(const Cd (*U)[kNB], int nb, int unit, int g,
-                                          Held &h) {
+template <bool LEFT, bool FORWARD, bool FULL, bool UNIT, int I>
+__device__ __forceinline__ void leaf_step(const StepCoef &k, int nb, int g, Held &h) {
   constexpr int i = FORWARD ? I : kNB - 1 - I;
   constexpr int go = i % kRowLanes, jo = i / kRowLanes;
-  if (i >= nb) return;  // uniform: the block is shorter than 64
+  if constexpr (!FULL) {
+    if (i >= nb) return;  // uniform: the block is shorter than 64
+  }
   const Cd v = h.at<jo>();
   Cd xi = v;
-  if (!unit) xi = cdiv(v, U[i][i]);
+  if constexpr (!UNIT) xi = cmul(v, k.dinv);
   xi.re = row_bcast<go>(xi.re);
   xi.im = row_bcast<go>(xi.im);
-  leaf_update<LEFT, FORWARD, I, 0>(U, g, xi, h);
-  leaf_update<LEFT, FORWARD, I, 1>(U, g, xi, h);
-  leaf_update<LEFT, FORWARD, I, 2>(U, g, xi, h);
-  leaf_update<LEFT, FORWARD, I, 3>(U, g, xi, h);
+  leaf_update<LEFT, FORWARD, I, 0>(k, g, xi, h);
+  leaf_update<LEFT, FORWARD, I, 1>(k, g, xi, h);
+  leaf_update<LEFT, FORWARD, I, 2>(k, g, xi, h);
+  leaf_update<LEFT, FORWARD, I, 3>(k, g, xi, h);
 }
 
-template <bool LEFT, bool FORWARD, int I>
+// Steps I .. 63, step I's coefficients in `cur`. FULL (a 64-wide block) and
+// UNIT (unit diagonal) are compile-time so that the unrolled steps are
+// straight-line code.
+template <bool LEFT, bool FORWARD, bool FULL, bool UNIT, int I>
 struct LeafRun {
-  __device__ __forceinline__ static void go(const Cd (*U)[kNB], int nb, int unit, int g,
-                                            Held &h) {
-    leaf_step<LEFT, FORWARD, I>(U, nb, unit, g, h);
-    LeafRun<LEFT, FORWARD, I + 1>::go(U, nb, unit, g, h);
+  __device__ __forceinline__ static void go(const Cd (*U)[kNB], const Cd *dinv, int nb, int g,
+                                            Held &h, const StepCoef &cur) {
+    StepCoef nxt;
+    load_coef<FORWARD, UNIT, I + 1>(U, dinv, g, nxt);
+    leaf_step<LEFT, FORWARD, FULL, UNIT, I>(cur, nb, g, h);
+    LeafRun<LEFT, FORWARD, FULL, UNIT, I + 1>::go(U, dinv, nb, g, h, nxt);
   }
 };
-template <bool LEFT, bool FORWARD>
-struct LeafRun<LEFT, FORWARD, kNB> {
-  __device__ __forceinline__ static void go(const Cd (*)[kNB], int, int, int, Held &) {}
+template <bool LEFT, bool FORWARD, bool FULL, bool UNIT>
+struct LeafRun<LEFT, FORWARD, FULL, UNIT, kNB> {
+  __device__ __forceinline__ static void go(const Cd (*)[kNB], const Cd *, int, int, Held &,
+                                            const StepCoef &) {}
 };
 
 // B entry r of right-hand side `rhs`: row lo + r of column rhs (Left) or
@@ -181,10 +222,10 @@ __device__ __forceinline__ void leaf_store(T *b, int64_t lo, int nb, int64_t rhs
 
 // Solves the diagonal block [lo, lo + nb) of op(A) for `nrhs` right-hand
 // sides. FORWARD: Left + effective lower, or Right + effective upper.
-template <typename T, bool LEFT, bool FORWARD>
+template <typename T, bool LEFT, bool FORWARD, bool FULL, bool UNIT>
 __global__ __launch_bounds__(kLeafThreads) void ctrsm_leaf(const T *a, int64_t lda, int trans,
-                                                           int unit, int64_t lo, int nb,
-                                                           int64_t nrhs, T *b, int64_t ldb) {
+                                                           int64_t lo, int nb, int64_t nrhs, T *b,
+                                                           int64_t ldb) {
   // U[i][r]: coefficient of x_i in the update of entry r (op(A)(r, i) for
   // Left, op(A)(i, r) for Right), for r after i in substitution order;
   // U[i][i] the diagonal; zero elsewhere.
@@ -201,6 +242,17 @@ __global__ __launch_bounds__(kLeafThreads) void ctrsm_leaf(const T *a, int64_t l
     const double imv = trans == 2 ? 0.0 - (double)im : (double)im;
     U[i][r] = Cd{in ? (double)re : 0.0, in ? imv : 0.0};
   }
+  // 1 / op(A)(i, i) = conj(a_ii) / norm_sqr(a_ii): a step multiplies by it
+  // where rblas divides (complex3.rs:344-349).
+  __shared__ Cd dinv[kNB];
+  if constexpr (!UNIT) {
+    __syncthreads();
+    if (threadIdx.x < kNB) {
+      const Cd d = U[threadIdx.x][threadIdx.x];
+      const double n = __builtin_fma(d.re, d.re, d.im * d.im);
+      dinv[threadIdx.x] = Cd{d.re / n, (0.0 - d.im) / n};
+    }
+  }
   const int g = threadIdx.x % kRowLanes;
   const int64_t rhs = blockIdx.x * (int64_t)kRhsPerBlock + threadIdx.x / kRowLanes;
   const bool on = rhs < nrhs;
@@ -211,7 +263,9 @@ __global__ __launch_bounds__(kLeafThreads) void ctrsm_leaf(const T *a, int64_t l
   leaf_load<T, LEFT, 2>(b, lo, nb, rhs, on, ldb, safe, g, h);
   leaf_load<T, LEFT, 3>(b, lo, nb, rhs, on, ldb, safe, g, h);
   __syncthreads();
-  LeafRun<LEFT, FORWARD, 0>::go(U, nb, unit, g, h);
+  StepCoef first;
+  load_coef<FORWARD, UNIT, 0>(U, dinv, g, first);
+  LeafRun<LEFT, FORWARD, FULL, UNIT, 0>::go(U, dinv, nb, g, h, first);
   leaf_store<T, LEFT, 0>(b, lo, nb, rhs, on, ldb, g, h);
   leaf_store<T, LEFT, 1>(b, lo, nb, rhs, on, ldb, g, h);
   leaf_store<T, LEFT, 2>(b, lo, nb, rhs, on, ldb, g, h);
@@ -237,13 +291,21 @@ struct CSolver {
   hipStream_t st;
   bool upper, forward;
 
+  template <bool LEFT, bool FORWARD, bool FULL, bool UNIT>
+  hipError_t leaf_launch(int64_t lo, int nb, int64_t nrhs) {
+    hipLaunchKernelGGL((ctrsm_leaf<T, LEFT, FORWARD, FULL, UNIT>),
+                       dim3((unsigned)((nrhs + kRhsPerBlock - 1) / kRhsPerBlock)),
+                       dim3(kLeafThreads), 0, st, p.a, p.lda, p.trans, lo, nb, nrhs, p.b, p.ldb);
+    return hipGetLastError();
+  }
+
   template <bool LEFT, bool FORWARD>
   hipError_t leaf_as(int64_t lo, int nb, int64_t nrhs) {
-    hipLaunchKernelGGL((ctrsm_leaf<T, LEFT, FORWARD>),
-                       dim3((unsigned)((nrhs + kRhsPerBlock - 1) / kRhsPerBlock)),
-                       dim3(kLeafThreads), 0, st, p.a, p.lda, p.trans, (int)p.unit, lo, nb, nrhs,
-                       p.b, p.ldb);
-    return hipGetLastError();
+    if (nb == kNB)
+      return p.unit ? leaf_launch<LEFT, FORWARD, true, true>(lo, nb, nrhs)
+                    : leaf_launch<LEFT, FORWARD, true, false>(lo, nb, nrhs);
+    return p.unit ? leaf_launch<LEFT, FORWARD, false, true>(lo, nb, nrhs)
+                  : leaf_launch<LEFT, FORWARD, false, false>(lo, nb, nrhs);
   }
 
   hipError_t leaf(int64_t lo, int nb) {

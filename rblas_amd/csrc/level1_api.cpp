@@ -157,7 +157,9 @@ int32_t dot_dev(size_t n, const T *x, size_t x_len, size_t incx, const T *y, siz
   if (!c.ok()) return fail(c, nullptr);
   hipStream_t st = (hipStream_t)stream;
   if (n == 0) {
-    RBLAS_HIP_TRY(hipMemsetAsync(result, 0, sizeof(T), st), "dot zero result");
+    // A store kernel, not hipMemsetAsync: callers may capture this call in
+    // a HIP graph, where memset nodes are avoided.
+    RBLAS_HIP_TRY(launch_scale<T>(1, 1, T(0), result, 1, st), "dot zero result");
     return RBLAS_OK;
   }
   void *scratch = nullptr;
@@ -174,7 +176,7 @@ int32_t nrm2_dev(size_t n, const T *x, size_t x_len, size_t incx, T *result, voi
   if (!c.ok()) return fail(c, nullptr);
   hipStream_t st = (hipStream_t)stream;
   if (n == 0) {
-    RBLAS_HIP_TRY(hipMemsetAsync(result, 0, sizeof(T), st), "nrm2 zero result");
+    RBLAS_HIP_TRY(launch_scale<T>(1, 1, T(0), result, 1, st), "nrm2 zero result");
     return RBLAS_OK;
   }
   void *scratch = nullptr;

@@ -140,4 +140,13 @@ def test_level1_device_entries(gpu, dtype):
     yh = y.copy()
     level1.axpy(n, 1.25, x, 1, yh, 1)
     assert np.array_equal(dy.download(s), yh)
+    # n == 0: rblas returns 0; the device result is stored as 0 (over a NaN).
+    r = DeviceArray.from_host(np.array([np.nan], dtype), s)
+    level1.dot_dev(0, dx, 1, dy, 1, r, s)
+    s.synchronize()
+    assert r.download(s)[0] == 0
+    r = DeviceArray.from_host(np.array([np.nan], dtype), s)
+    level1.nrm2_dev(0, dx, 1, r, s)
+    s.synchronize()
+    assert r.download(s)[0] == 0
     s.close()
