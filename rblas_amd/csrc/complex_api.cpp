@@ -235,6 +235,9 @@ Check ctrsm_validate(int32_t side, int32_t uplo, int32_t trans, int32_t diag, si
   const size_t q = side == RBLAS_LEFT ? m : n;
   if ((2 * q * 128 + 160) * sizeof(T) >= 0x7fffffe0ull)
     return bad_arg("complex trsm: order too large for the device path");
+  // The updates run as real gemms on B viewed with leading dimension 2 ldb.
+  if ((2 * ldb * 128 + 160) * sizeof(T) >= 0x7fffffe0ull)
+    return bad_arg("complex trsm: ldb too large for the device path");
   return Check{};
 }
 

@@ -5,8 +5,11 @@
 //   1. B := alpha B when alpha != 1 (rblas's complex multiply, every entry).
 //   2. The diagonal range is solved by recursive halving: solve the half that
 //      comes first in substitution order, subtract its contribution from the
-//      other half with one 4M complex gemm (alpha -1, beta 1), solve the other
-//      half.
+//      other half, solve the other half, by real MFMA gemms on the real view
+//      of B (its rows alternating re / im, ld 2 ldb). Left: one gemm with the
+//      real representation of op(A) (realrep_kernel, formed once per call);
+//      Right: X re(op A) + (i X) im(op A), two gemms with op(A)'s planes
+//      (formed once) and a rotated copy of X (rot_kernel, per update).
 //   3. A range of up to 64 is solved by substitution in f64, 16 lanes per
 //      right-hand side (a column of B for Left, a row for Right), the block
 //      of op(A) in LDS: s = s - a*x with rblas's complex multiply, then
@@ -55,12 +58,12 @@ __global__ void cscale_kernel(int64_t m, int64_t n, T ar, T ai, T *b, int64_t ld
 // Leaves: one 16-lane DPP row per right-hand side (a column of B for Left,
 // a row for Right), each lane holding entries r = g + 16 jj (g its lane in
 // the row, jj < 4) in registers. The solve is right-looking: at step i the
-// lane that holds entry i divides it by the diagonal, the quotient x_i is
-// broadcast across the row, and every later entry r subtracts
-// coef(i, r) x_i. An entry receives its subtractions in rblas's order (k
-// ascending) in a forward solve and in descending k in a backward one; the
-// entries a product with a non-finite x_i reaches are the ones rblas's sums
-// reach (every k in the triangle, zero coefficients included). Steps are unrolled so that the
+// lane that holds entry i scales it by 1 / a_ii, the result x_i is broadcast
+// across the row, and every later entry r subtracts coef(i, r) x_i. An entry
+// receives its subtractions in rblas's order (k ascending) in a forward solve
+// and in descending k in a backward one; the entries a product with a
+// non-finite x_i reaches are the ones rblas's sums reach (every k in the
+// triangle, zero coefficients included). Steps are unrolled so that the
 // owning lane and register are compile-time constants.
 constexpr int kRowLanes = 16;                 // lanes per right-hand side (a DPP row)
 constexpr int kPer = kNB / kRowLanes;         // entries held per lane
@@ -279,15 +282,82 @@ unsigned grid_for(int64_t total) {
   return (unsigned)g;
 }
 
-// Offset (in complex elements) of op(A)(r0, c0) as a gemm operand.
-inline int64_t sub_off(int64_t lda, bool trans, int64_t r0, int64_t c0) {
-  return trans ? c0 + r0 * lda : r0 + c0 * lda;
+// PLANES: op(A)'s real and imaginary parts as two q x q real matrices (ld q,
+// the imaginary one after the real one). Otherwise the real representation
+// of op(A) (q x q complex) as a 2q x 2q real column-major matrix R (ld 2q): R(2r + a, 2c + b) = [[re, -im], [im, re]][a][b]
+// of op(A)(r, c), conjugated for ConjTrans. An interleaved complex B is the
+// real 2m x n matrix of its rows (re, im alternating, ld 2 ldb), and
+// R_sub X_view is then exactly the real view of op(A)_sub X: one real gemm
+// per update, in place on B. 32 x 32 tiles of op(A); for Trans / ConjTrans
+// the source tile is read along A's columns and transposed through LDS.
+constexpr int kRepTile = 32, kRepRows = 8;
+
+template <typename T, bool PLANES>
+__global__ __launch_bounds__(kRepTile *kRepRows) void realrep_kernel(const T *a, int64_t lda,
+                                                                     int trans, int64_t q, T *out) {
+  __shared__ Cd s[kRepTile][kRepTile + 1];
+  const int64_t r0 = (int64_t)blockIdx.x * kRepTile, c0 = (int64_t)blockIdx.y * kRepTile;
+  const int tx = threadIdx.x, ty = threadIdx.y;
+  if (trans) {
+    // A(c, r) for c = c0 + tx (contiguous in A), r = r0 + ty + 8k.
+#pragma unroll
+    for (int k = 0; k < kRepTile / kRepRows; ++k) {
+      const int64_t c = c0 + tx, r = r0 + ty + kRepRows * k;
+      if (c < q && r < q) {
+        const T *v = a + 2 * (c + r * lda);
+        s[ty + kRepRows * k][tx] = Cd{(double)v[0], (double)v[1]};
+      }
+    }
+    __syncthreads();
+  }
+  const int64_t ld = 2 * q;
+#pragma unroll
+  for (int k = 0; k < kRepTile / kRepRows; ++k) {
+    const int64_t r = r0 + tx, c = c0 + ty + kRepRows * k;
+    if (r >= q || c >= q) continue;
+    T re, im;
+    if (trans) {
+      const Cd v = s[tx][ty + kRepRows * k];
+      re = (T)v.re;
+      im = trans == 2 ? T(0) - (T)v.im : (T)v.im;
+    } else {
+      const T *v = a + 2 * (r + c * lda);
+      re = v[0];
+      im = v[1];
+    }
+    if constexpr (PLANES) {
+      out[r + c * q] = re;
+      out[q * q + r + c * q] = im;
+    } else {
+      T *o = out + 2 * r + 2 * c * ld;
+      o[0] = re;
+      o[1] = im;
+      o[ld] = T(0) - im;
+      o[ld + 1] = re;
+    }
+  }
+}
+
+// W (real 2m x cols, ld 2m) := the real view of i X for X = B's columns
+// [c0, c0 + cols): W(2i, j) = -im X(i, j), W(2i + 1, j) = re X(i, j).
+template <typename T>
+__global__ void rot_kernel(int64_t m, int64_t cols, const T *x, int64_t ldx, T *w) {
+  const int64_t total = m * cols;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t j = e / m, i = e - j * m;
+    const T *v = x + 2 * (i + j * ldx);
+    T *o = w + 2 * (i + j * m);
+    const T re = v[0], im = v[1];
+    o[0] = T(0) - im;
+    o[1] = re;
+  }
 }
 
 template <typename T>
 struct CSolver {
   const CtrsmProblem<T> &p;
-  T *ws;  // complex gemm workspace
+  T *ws;  // Left: the real representation of op(A); Right: op(A)'s planes, then i X
   hipStream_t st;
   bool upper, forward;
 
@@ -326,19 +396,33 @@ struct CSolver {
     }
     hipError_t e = range(f_lo, f_len);
     if (e != hipSuccess) return e;
-    const bool tr = p.trans != 0;
     if (p.left) {
-      // B[second rows] -= op(A)[second, first] X[first rows]
-      CgemmProblem<T> u{p.trans, 0, s_len, p.n, f_len, T(-1), T(0), T(1), T(0),
-                        p.a + 2 * sub_off(p.lda, tr, s_lo, f_lo), p.lda, p.b + 2 * f_lo, p.ldb,
-                        p.b + 2 * s_lo, p.ldb};
-      e = launch_cgemm<T>(u, ws, st);
+      // B[second rows] -= op(A)[second, first] X[first rows], on real views:
+      // rows 2 s_lo.. of B -= R[2 s_lo.., 2 f_lo..] rows 2 f_lo.. of B.
+      const int64_t q2 = 2 * p.m;
+      GemmProblem<T> u{false, false, 2 * s_len, p.n, 2 * f_len, T(-1), T(1),
+                       ws + 2 * s_lo + 2 * f_lo * q2, q2, p.b + 2 * f_lo, 2 * p.ldb,
+                       p.b + 2 * s_lo, 2 * p.ldb};
+      e = launch_gemm<T>(u, st);
     } else {
-      // B[:, second] -= X[:, first] op(A)[first, second]
-      CgemmProblem<T> u{0, p.trans, p.m, s_len, f_len, T(-1), T(0), T(1), T(0),
-                        p.b + 2 * f_lo * p.ldb, p.ldb, p.a + 2 * sub_off(p.lda, tr, f_lo, s_lo),
-                        p.lda, p.b + 2 * s_lo * p.ldb, p.ldb};
-      e = launch_cgemm<T>(u, ws, st);
+      // B[:, second] -= X[:, first] op(A)[first, second] on real views (rows
+      // of B alternating re / im): X re(op A) + (i X) im(op A).
+      const int64_t q = p.n, m2 = 2 * p.m;
+      const T *pre = ws, *pim = ws + q * q;
+      T *wrot = ws + 2 * q * q;
+      const T *x = p.b + 2 * f_lo * p.ldb;
+      hipLaunchKernelGGL((rot_kernel<T>), dim3(grid_for(p.m * f_len)), dim3(256), 0, st, p.m,
+                         f_len, x, p.ldb, wrot);
+      e = hipGetLastError();
+      if (e != hipSuccess) return e;
+      T *c = p.b + 2 * s_lo * p.ldb;
+      GemmProblem<T> g1{false, false, m2, s_len, f_len, T(-1), T(1),
+                        x, 2 * p.ldb, pre + f_lo + s_lo * q, q, c, 2 * p.ldb};
+      e = launch_gemm<T>(g1, st);
+      if (e != hipSuccess) return e;
+      GemmProblem<T> g2{false, false, m2, s_len, f_len, T(-1), T(1),
+                        wrot, m2, pim + f_lo + s_lo * q, q, c, 2 * p.ldb};
+      e = launch_gemm<T>(g2, st);
     }
     if (e != hipSuccess) return e;
     return range(s_lo, s_len);
@@ -348,8 +432,7 @@ struct CSolver {
 }  // namespace
 
 size_t ctrsm_workspace_elems(int64_t m, int64_t n, bool left) {
-  const int64_t q = left ? m : n;
-  return left ? cgemm_workspace_elems(q, n, q) : cgemm_workspace_elems(m, q, q);
+  return left ? (size_t)(4 * m * m) : (size_t)(2 * n * n + 2 * m * n);
 }
 
 template <typename T>
@@ -358,6 +441,18 @@ hipError_t launch_ctrsm(const CtrsmProblem<T> &p, T *ws, hipStream_t st) {
   if (!(p.alpha_re == T(1) && p.alpha_im == T(0)))
     hipLaunchKernelGGL((cscale_kernel<T>), dim3(grid_for(p.m * p.n)), dim3(256), 0, st, p.m, p.n,
                        p.alpha_re, p.alpha_im, p.b, p.ldb);
+  const int64_t q = p.left ? p.m : p.n;
+  if (q > kNB) {
+    const unsigned t = (unsigned)((q + kRepTile - 1) / kRepTile);
+    if (p.left)
+      hipLaunchKernelGGL((realrep_kernel<T, false>), dim3(t, t), dim3(kRepTile, kRepRows), 0, st,
+                         p.a, p.lda, p.trans, q, ws);
+    else
+      hipLaunchKernelGGL((realrep_kernel<T, true>), dim3(t, t), dim3(kRepTile, kRepRows), 0, st,
+                         p.a, p.lda, p.trans, q, ws);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
   const bool tr = p.trans != 0;
   const bool upper = p.upper != tr;  // effective triangle of op(A)
   // Left+lower and Right+upper substitute from index 0 up, the others down.
