@@ -243,10 +243,11 @@ int32_t rblas_dtrsm_dev(int32_t side, int32_t uplo, int32_t trans, int32_t diag,
  * Replaces `complex3::trsm` (complex3.rs:252-349): the same checks as the
  * real trsm (nothing when m or n is 0; RBLAS_PANIC at B's largest index
  * (m-1) + (n-1)*ldb, then op(A)'s diagonal (q-1)*(lda+1)); RBLAS_BAD_ARG for
- * ldb < m with n > 1. alpha != 1 pre-scales B; NonUnit divides by
- * rblas's cdiv (s conj(a_ii) / norm_sqr(a_ii)). Computed by recursive
- * halving: real-view MFMA gemm updates and 64-wide substitution leaves in
- * f64.
+ * ldb < m with n > 1. alpha != 1 pre-scales B; NonUnit multiplies by
+ * 1 / a_ii = conj(a_ii) / norm_sqr(a_ii), formed once per diagonal block,
+ * where rblas's cdiv divides s conj(a_ii) by norm_sqr(a_ii) (the same
+ * quotient up to rounding). Computed by recursive halving: real-view MFMA
+ * gemm updates and 64-wide substitution leaves in f64.
  * Lengths and leading dimensions count complex elements. */
 int32_t rblas_ctrsm(int32_t side, int32_t uplo, int32_t trans, int32_t diag, size_t m,
                     size_t n, rblas_complex32_t alpha, const rblas_complex32_t *a,
