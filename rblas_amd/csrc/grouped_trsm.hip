@@ -99,13 +99,15 @@ __device__ __forceinline__ void invert_block(const TrsmDesc<T> &d, int blk, doub
   for (int c = 0; c < 4; ++c) x[c] = lane == j0 + c ? 1.0 : 0.0;
   for (int k = j0; k < kNB; ++k) {
     const double lik = Ls[k][lane];
-    const double rk = lane_value(rl, k);
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
       if (k >= j0 + c) {  // wave-uniform
-        const double xk = lane_value(x[c], k) * rk;
+        // Every lane forms x / L(lane, lane); lane k's is x_k / L(k, k), the
+        // value broadcast (the same product the row-wise substitution forms).
+        const double y = x[c] * rl;
+        const double xk = lane_value(y, k);
         const double upd = __builtin_fma(-lik, xk, x[c]);
-        x[c] = lane == k ? xk : (lane > k ? upd : x[c]);
+        x[c] = lane == k ? y : (lane > k ? upd : x[c]);
       }
     }
   }
