@@ -113,7 +113,7 @@ int32_t cgemm_dev(int32_t ta, int32_t tb, size_t m, size_t n, size_t k, T ar, T 
   CgemmProblem<T> p{ta, tb, (int64_t)m, (int64_t)n, (int64_t)k, ar, ai, br, bi, a, (int64_t)lda,
                     b, (int64_t)ldb, c, (int64_t)ldc};
   hipError_t e = launch_cgemm<T>(p, (T *)ws, st);
-  hipFreeAsync(ws, st);
+  e = pool_release(ws, st, e);
   RBLAS_HIP_TRY(e, "complex gemm launch");
   return RBLAS_OK;
 }
@@ -217,7 +217,7 @@ int32_t hesy_dev(bool herm, int32_t side, int32_t uplo, size_t m, size_t n, T ar
                 "symm/hemm workspace");
   hipError_t e = hesy_enqueue<T>(left, uplo == RBLAS_UPPER, herm, m, n, ar, ai, a, lda, b, ldb,
                                  br, bi, c, ldc, (T *)ws, st);
-  hipFreeAsync(ws, st);
+  e = pool_release(ws, st, e);
   RBLAS_HIP_TRY(e, "symm/hemm launch");
   return RBLAS_OK;
 }
@@ -296,7 +296,7 @@ int32_t ctrsm_dev(int32_t side, int32_t uplo, int32_t trans, int32_t diag, size_
   const CtrsmProblem<T> p = ctrsm_problem<T>(side, uplo, trans, diag, m, n, ar, ai, a, lda, b,
                                              ldb);
   hipError_t e = launch_ctrsm<T>(p, (T *)ws, st);
-  hipFreeAsync(ws, st);
+  e = pool_release(ws, st, e);
   RBLAS_HIP_TRY(e, "complex trsm launch");
   return RBLAS_OK;
 }

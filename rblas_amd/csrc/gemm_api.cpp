@@ -39,8 +39,9 @@ size_t workspace_bytes(const ColMajorCall<T> &cc, int planes) {
   return gemm_workspace_bytes<T>(p, gemm_config<T>(p));
 }
 
+// f64: no split compute modes (`planes` is used by the f32 form only).
 template <typename T>
-hipError_t launch_product(const GemmProblem<T> &p, int planes, void *ws, hipStream_t st) {
+hipError_t launch_product(const GemmProblem<T> &p, int /*planes*/, void *ws, hipStream_t st) {
   return launch_gemm_cfg<T>(p, gemm_config<T>(p), ws, st);
 }
 template <>
@@ -147,7 +148,7 @@ int32_t gemm_dev_entry(int32_t layout, int32_t ta, int32_t tb, size_t m, size_t 
   void *ws = nullptr;
   if (wsb) RBLAS_HIP_TRY(pool_alloc(&ws, wsb, st), "gemm workspace");
   hipError_t e = enqueue(cc, cc.a, cc.lda, cc.b, cc.ldb, cc.c, cc.ldc, planes, ws, st);
-  if (ws) hipFreeAsync(ws, st);
+  e = pool_release(ws, st, e);
   RBLAS_HIP_TRY(e, "gemm launch");
   return RBLAS_OK;
 }
@@ -275,7 +276,7 @@ int32_t symm_dev(int32_t side, int32_t uplo, size_t m, size_t n, T alpha, const 
   if (wsb) RBLAS_HIP_TRY(pool_alloc(&ws, wsb, st), "symm workspace");
   hipError_t e = symm_enqueue<T>(cc, left, uplo == RBLAS_UPPER, a, lda, b, ldb, c, ldc, (T *)ws,
                                  ws ? (char *)ws + full_b : nullptr, st);
-  if (ws) hipFreeAsync(ws, st);
+  e = pool_release(ws, st, e);
   RBLAS_HIP_TRY(e, "symm launch");
   return RBLAS_OK;
 }

@@ -98,7 +98,8 @@ int32_t grouped_gemm_create(rblas_grouped_t **out, const P *probs, size_t count,
   const GemmGroup<T> &kept = gemm_of<T>(g);
   const int32_t s = upload(g, kept, grouped_gemm_grid<T>());
   if (s != RBLAS_OK) {
-    if (g->mem) hipFree(g->mem);
+    // The upload's failure is the one reported; the free is best effort.
+    if (g->mem) static_cast<void>(hipFree(g->mem));
     delete g;
     return s;
   }
@@ -138,7 +139,8 @@ int32_t grouped_trsm_create(rblas_grouped_t **out, const P *probs, size_t count,
   trsm_of<T>(g) = std::move(grp);
   const int32_t s = upload(g, trsm_of<T>(g), grouped_trsm_grid<T>());
   if (s != RBLAS_OK) {
-    if (g->mem) hipFree(g->mem);
+    // The upload's failure is the one reported; the free is best effort.
+    if (g->mem) static_cast<void>(hipFree(g->mem));
     delete g;
     return s;
   }
@@ -199,7 +201,8 @@ int32_t rblas_grouped_run_graph(rblas_grouped_t *plan, void *stream) {
     hipGraph_t graph = nullptr;
     const hipError_t e = hipStreamEndCapture(st, &graph);
     if (s != RBLAS_OK) {
-      if (graph) hipGraphDestroy(graph);
+      // The launch's failure is the one reported; the destroy is best effort.
+      if (graph) static_cast<void>(hipGraphDestroy(graph));
       return s;
     }
     RBLAS_HIP_TRY(e, "graph capture end");
@@ -218,9 +221,12 @@ int32_t rblas_grouped_info(const rblas_grouped_t *plan, rblas_grouped_info_t *in
 
 int32_t rblas_grouped_destroy(rblas_grouped_t *plan) {
   if (!plan) return RBLAS_OK;
-  if (plan->exec) hipGraphExecDestroy(plan->exec);
-  if (plan->graph) hipGraphDestroy(plan->graph);
-  hipError_t e = plan->mem ? hipFree(plan->mem) : hipSuccess;
+  // Everything is released; the first failure is reported.
+  hipError_t e = plan->exec ? hipGraphExecDestroy(plan->exec) : hipSuccess;
+  const hipError_t eg = plan->graph ? hipGraphDestroy(plan->graph) : hipSuccess;
+  const hipError_t em = plan->mem ? hipFree(plan->mem) : hipSuccess;
+  if (e == hipSuccess) e = eg;
+  if (e == hipSuccess) e = em;
   delete plan;
   RBLAS_HIP_TRY(e, "grouped destroy");
   return RBLAS_OK;

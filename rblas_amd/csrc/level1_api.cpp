@@ -165,7 +165,7 @@ int32_t dot_dev(size_t n, const T *x, size_t x_len, size_t incx, const T *y, siz
   void *scratch = nullptr;
   RBLAS_HIP_TRY(pool_alloc(&scratch, level1_scratch_bytes(), st), "dot scratch");
   hipError_t e = launch_dot<T>((int64_t)n, x, (int64_t)incx, y, (int64_t)incy, result, scratch, st);
-  hipFreeAsync(scratch, st);
+  e = pool_release(scratch, st, e);
   RBLAS_HIP_TRY(e, "dot launch");
   return RBLAS_OK;
 }
@@ -182,7 +182,7 @@ int32_t nrm2_dev(size_t n, const T *x, size_t x_len, size_t incx, T *result, voi
   void *scratch = nullptr;
   RBLAS_HIP_TRY(pool_alloc(&scratch, level1_scratch_bytes(), st), "nrm2 scratch");
   hipError_t e = launch_nrm2<T>((int64_t)n, x, (int64_t)incx, result, scratch, st);
-  hipFreeAsync(scratch, st);
+  e = pool_release(scratch, st, e);
   RBLAS_HIP_TRY(e, "nrm2 launch");
   return RBLAS_OK;
 }

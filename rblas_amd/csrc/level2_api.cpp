@@ -187,7 +187,7 @@ int32_t gemv_dev(bool cplx, int32_t layout, int32_t trans, size_t m, size_t n, T
   void *ws = nullptr;
   if (wsb > 0) RBLAS_HIP_TRY(pool_alloc(&ws, wsb, st), "gemv workspace");
   hipError_t e = launch_gemv<T>(p, ws, st);
-  if (ws) hipFreeAsync(ws, st);
+  e = pool_release(ws, st, e);
   RBLAS_HIP_TRY(e, "gemv launch");
   return RBLAS_OK;
 }

@@ -50,14 +50,20 @@ int32_t hip_fail(hipError_t e, const char *what) {
   return e == hipErrorOutOfMemory ? RBLAS_ALLOC_FAILED : RBLAS_HIP_ERROR;
 }
 
+hipError_t pool_release(void *ptr, hipStream_t stream, hipError_t prior) {
+  const hipError_t e = ptr ? hipFreeAsync(ptr, stream) : hipSuccess;
+  return prior != hipSuccess ? prior : e;
+}
+
 hipError_t Ctx::ensure(int slot, size_t bytes) {
   if (bytes <= cap[slot]) return hipSuccess;
   if (buf[slot]) {
     hipError_t e = hipStreamSynchronize(stream);
     if (e != hipSuccess) return e;
-    hipFree(buf[slot]);
+    e = hipFree(buf[slot]);
     buf[slot] = nullptr;
     cap[slot] = 0;
+    if (e != hipSuccess) return e;
   }
   size_t want = bytes < 4096 ? 4096 : bytes;
   hipError_t e = hipMalloc(&buf[slot], want);
@@ -74,9 +80,10 @@ hipError_t Ctx::ensure_pinned(size_t bytes) {
   if (pinned) {
     hipError_t e = hipStreamSynchronize(stream);
     if (e != hipSuccess) return e;
-    hipHostFree(pinned);
+    e = hipHostFree(pinned);
     pinned = nullptr;
     pinned_cap = 0;
+    if (e != hipSuccess) return e;
   }
   size_t want = bytes < (size_t(1) << 20) ? (size_t(1) << 20) : bytes;
   hipError_t e = hipHostMalloc((void **)&pinned, want, hipHostMallocDefault);

@@ -18,6 +18,10 @@ void set_error(const std::string &msg);
 // pool, which is configured once to keep freed memory cached (no release to
 // the OS between calls). Free with hipFreeAsync on the same stream.
 hipError_t pool_alloc(void **ptr, size_t bytes, hipStream_t stream);
+// Returns `ptr` (from pool_alloc, or null) to the pool once the work enqueued
+// on `stream` before it completes. The result is `prior` when that is an
+// error (the launch's failure is the one to report), else the free's.
+hipError_t pool_release(void *ptr, hipStream_t stream, hipError_t prior);
 int32_t fail(const Check &c, rblas_error_t *err);
 int32_t hip_fail(hipError_t e, const char *what);
 

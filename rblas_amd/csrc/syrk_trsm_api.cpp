@@ -151,7 +151,7 @@ int32_t trsm_dev(int32_t side, int32_t uplo, int32_t trans, int32_t diag, size_t
                 "trsm workspace");
   TrsmProblem<T> p = trsm_problem<T>(side, uplo, trans, diag, m, n, alpha, a, lda, b, ldb);
   hipError_t e = launch_trsm<T>(p, ws, st);
-  hipFreeAsync(ws, st);
+  e = pool_release(ws, st, e);
   RBLAS_HIP_TRY(e, "trsm launch");
   return RBLAS_OK;
 }
