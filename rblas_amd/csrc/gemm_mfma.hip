@@ -8,7 +8,7 @@
 //     loads (`buffer_load_dwordx4 ... lds`), four LDS stages in flight, one
 //     barrier per k-tile, no staging registers;
 //   - the MR x NR register microtile -> a per-wave block of 16 x 16 MFMA
-//     accumulators (64 x 64 per wave in a 128 x 128 tile);
+//     accumulators (64 x 32 per wave in a 128 x 128 tile, 8 waves);
 //   - the jc/pc/ic loops -> one workgroup per C tile (and per k split when
 //     the tile grid alone cannot fill the 256 CUs), the k range accumulated
 //     in registers;

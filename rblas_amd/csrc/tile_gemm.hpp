@@ -2,14 +2,15 @@
 // the grouped persistent GEMM (grouped.hip). Device code only; every
 // translation unit that includes it gets its own LDS stages.
 //
-// One workgroup (4 waves, 2 x 2) computes one BM x BN tile of
+// One workgroup (8 waves, 2 x 4: two waves per SIMD, so one wave's barrier
+// and LDS waits overlap the other's MFMAs) computes one BM x BN tile of
 // C := alpha * op(A) * op(B) + beta * C (column-major), BM, BN in {64, 128},
-// each wave a (BM/2) x (BN/2) block of 16 x 16 MFMA accumulators, over a
+// each wave a (BM/2) x (BN/4) block of 16 x 16 MFMA accumulators, over a
 // k range [kbeg, kend) (the whole k, or one split of it).
 //
 // LDS staging. One DMA instruction moves 64 lanes x 16 bytes = 64*V elements
 // (V = 16 / sizeof(T): 2 for f64, 4 for f32) into one contiguous 1 KiB LDS
-// block, so a TILE x BK operand tile (BK = 8V) is TILE/8 blocks, TILE/32 per
+// block, so a TILE x BK operand tile (BK = 8V) is TILE/8 blocks, TILE/64 per
 // wave. Blocks are padded so that the four lane groups of a fragment read
 // fall on different banks:
 //   M/N-contiguous operand: block q holds tile k-rows q*R .. q*R+R-1
@@ -69,7 +70,8 @@ struct Mfma<float> {
   static __device__ __forceinline__ int row(int lane, int r) { return 4 * (lane >> 4) + r; }
 };
 
-constexpr int kThreads = 256;  // 4 waves of 64, 2 x 2 over the C tile
+constexpr int kWaves = 8;                // 2 x 4 over the C tile, two per SIMD
+constexpr int kThreads = 64 * kWaves;
 constexpr int kMaxTile = 128;  // largest C tile side
 constexpr int kStages = 4;     // LDS pipeline depth
 constexpr int kOperandBytes = (kMaxTile / 8) * (1024 + 64);  // largest operand stage
@@ -162,7 +164,7 @@ __device__ __forceinline__ int lds_byte(int k, int mn) {
 template <typename T, bool MN_MAJOR, int TILE>
 struct Operand {
   typedef Geo<T> G;
-  static constexpr int NB = TILE / 32;  // DMA blocks per wave per k-tile
+  static constexpr int NB = TILE / (8 * kWaves);  // DMA blocks per wave per k-tile
   static constexpr int R = 1024 / (TILE * (int)sizeof(T));
   const T *x;
   const T *end;  // one past the last element the stored operand may touch
@@ -305,8 +307,8 @@ struct TileGemm {
   static constexpr bool A_MN = !TA;   // A NoTrans is M-contiguous
   static constexpr bool B_MN = TB;    // B Trans is N-contiguous
   static constexpr int FM = BM / 32;  // 16 x 16 blocks per wave along m
-  static constexpr int FN = BN / 32;  // ... along n
-  static constexpr int kPerTile = BM / 32 + BN / 32;  // DMA instructions per wave per k-tile
+  static constexpr int FN = BN / (16 * (kWaves / 2));  // ... along n
+  static constexpr int kPerTile = BM / (8 * kWaves) + BN / (8 * kWaves);  // DMA per wave per k-tile
 
   Operand<T, A_MN, BM> a;
   Operand<T, B_MN, BN> b;
@@ -325,7 +327,7 @@ struct TileGemm {
     lane = tid & 63;
     const int wave = tid >> 6;
     wm = (wave & 1) * (BM / 2);
-    wn = (wave >> 1) * (BN / 2);
+    wn = (wave >> 1) * (BN / (kWaves / 2));
     kbeg = kb;
     kend = ke;
     ktiles = (ke - kb + G::BK - 1) / G::BK;
