@@ -12,10 +12,10 @@ namespace {
 
 // Defaults: the autotuner's measurement on MI355X (rblas_amd/autotune.py,
 // saved in rblas_amd/tuning_gfx950.json). The unit cap sits where splitting
-// stopped paying on the batching workload (for f64 a cap of 320 us and no
-// cap at all timed the same; 320 keeps very long-k units splittable).
-Tuning g_tuning = {{{4.746, 9.366}, {4.291, 10.249}},
-                   {{0.793, 2.305}, {0.734, 2.153}},
+// stopped paying on the batching workload (for f64 no cap timed 3% under a
+// cap of 320 us; 320 is kept so that very long-k units stay splittable).
+Tuning g_tuning = {{{4.189, 8.065}, {3.941, 10.431}},
+                   {{0.779, 2.188}, {0.751, 2.141}},
                    {320.0, 320.0}};
 std::mutex g_mu;
 
