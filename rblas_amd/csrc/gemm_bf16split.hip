@@ -16,9 +16,9 @@
 //                  layout, transposition and leading dimension and writes the
 //                  planes k-blocked by 16, zero-padded to 256 rows and to a
 //                  multiple of 48 k, so the product kernel needs no masking;
-//   split_gemm_kernel  256 x 256 C tile per workgroup, 4 waves of 128 x 128 as
-//                  4 x 4 v_mfma_f32_32x32x16_bf16 blocks (256 accumulator
-//                  registers); every plane's 16-k slice is moved global -> LDS
+//   split_gemm_kernel  256 x 256 C tile per workgroup, 8 waves (two per SIMD)
+//                  of 128 x 64 as 4 x 2 v_mfma_f32_32x32x16_bf16 blocks (128
+//                  accumulator registers); every plane's 16-k slice is moved global -> LDS
 //                  by buffer_load_dwordx4 ... lds (3 LDS stages, one barrier per
 //                  16 k) and each fragment is reused in registers by every
 //                  product it takes part in (96 MFMAs per barrier for BF16x6).
@@ -33,9 +33,12 @@ typedef __attribute__((address_space(3))) void *lds_void_ptr;
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
-constexpr int kThreads = 256;
+constexpr int kWaves = 8;              // 2 x 4 over the C tile, two per SIMD
+constexpr int kThreads = 64 * kWaves;
 constexpr int kTile = 256;             // C tile rows and columns; plane row padding
-constexpr int kWave = 128;             // per-wave C tile: 4 x 4 blocks of 32 x 32
+constexpr int kWave = 128;             // per-wave C tile rows: 4 blocks of 32
+constexpr int kWaveN = 64;             // per-wave C tile columns: 2 blocks of 32
+constexpr int kFN = kWaveN / 32;
 constexpr int kBK = 16;                // k per LDS stage (one MFMA k-step); plane k padding
 constexpr int kStages = 3;
 constexpr int kPlaneTileBytes = kTile * kBK * 2;  // 8 KB: 256 rows x 32 B
@@ -181,12 +184,12 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t block_rsrc(const char *base) {
 
 template <int PL>
 struct SplitGemm {
-  static constexpr int kPerWave = 2 * PL;        // DMA blocks (1 KB = 32 rows) per operand per wave
+  static constexpr int kPerWave = 8 * PL / kWaves;  // DMA blocks (1 KB = 32 rows) per operand per wave
   static constexpr int kPerTile = 2 * kPerWave;  // DMA instructions per wave per k-tile
   const char *ga[kPerWave];  // start of each of this wave's A blocks at k-block 0
   const char *gb[kPerWave];
   int64_t sa, sb;            // bytes from one k-block of a plane to the next
-  f32x16 acc[4][4];
+  f32x16 acc[4][kFN];
   int ktiles, lane, wave, wm, wn;
 
   __device__ __forceinline__ void init(const SplitArgs &p, int m0, int n0) {
@@ -229,14 +232,14 @@ struct SplitGemm {
     const char *As = lds_a<S>();
     const char *Bs = lds_b<S>();
     const int r32 = lane & 31, c = lane >> 5;
-    bf16x8 fa[PL][4], fb[PL][4];
+    bf16x8 fa[PL][4], fb[PL][kFN];
 #pragma unroll
-    for (int s = 0; s < PL; ++s)
+    for (int s = 0; s < PL; ++s) {
 #pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        fa[s][i] = *(const bf16x8 *)(As + lds_off(s, wm + 32 * i + r32, c));
-        fb[s][i] = *(const bf16x8 *)(Bs + lds_off(s, wn + 32 * i + r32, c));
-      }
+      for (int i = 0; i < 4; ++i) fa[s][i] = *(const bf16x8 *)(As + lds_off(s, wm + 32 * i + r32, c));
+#pragma unroll
+      for (int j = 0; j < kFN; ++j) fb[s][j] = *(const bf16x8 *)(Bs + lds_off(s, wn + 32 * j + r32, c));
+    }
     // Smallest terms first.
 #pragma unroll
     for (int pr = 0; pr < (PL == 2 ? 3 : 6); ++pr) {
@@ -253,7 +256,7 @@ struct SplitGemm {
 #pragma unroll
       for (int i = 0; i < 4; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j)
+        for (int j = 0; j < kFN; ++j)
           acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fb[sb][j], fa[sa][i], acc[i][j],
                                                               0, 0, 0);
     }
@@ -275,7 +278,7 @@ struct SplitGemm {
 
 // Workgroup u runs k split u / tiles of tile u % tiles. With splits, every
 // split stores its raw accumulators (register order: value e of thread t at
-// e * 256 + t of its slot), publishes them (device-scope release) and counts
+// e * kThreads + t of its slot), publishes them (device-scope release) and counts
 // itself on the tile's counter; the split that counts last (acquire) sums
 // the slots in split order -- a fixed order, so the bits do not depend on
 // which workgroup finishes last -- runs the epilogue and resets the counter.
@@ -305,7 +308,7 @@ __global__ __launch_bounds__(kThreads) void split_gemm_kernel(SplitArgs p) {
   g.wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   g.lane = tid & 63;
   g.wm = (g.wave & 1) * kWave;
-  g.wn = (g.wave >> 1) * kWave;
+  g.wn = (g.wave >> 1) * kWaveN;
   g.ktiles = p.Kp / kBK;
   const int kt0 = min(g.ktiles, split * p.split_ktiles);
   const int kt1 = min(g.ktiles, kt0 + p.split_ktiles);
@@ -313,7 +316,7 @@ __global__ __launch_bounds__(kThreads) void split_gemm_kernel(SplitArgs p) {
 #pragma unroll
   for (int i = 0; i < 4; ++i)
 #pragma unroll
-    for (int j = 0; j < 4; ++j) g.acc[i][j] = f32x16{};
+    for (int j = 0; j < kFN; ++j) g.acc[i][j] = f32x16{};
 
   if (kt0 < kt1) {
     // Every run is a multiple of kStages tiles; the two tiles prefetched past
@@ -334,9 +337,9 @@ __global__ __launch_bounds__(kThreads) void split_gemm_kernel(SplitArgs p) {
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
-      for (int j = 0; j < 4; ++j)
+      for (int j = 0; j < kFN; ++j)
 #pragma unroll
-        for (int r = 0; r < 16; ++r) mine[((i * 4 + j) * 16 + r) * kThreads] = g.acc[i][j][r];
+        for (int r = 0; r < 16; ++r) mine[((i * kFN + j) * 16 + r) * kThreads] = g.acc[i][j][r];
     __threadfence();
     __syncthreads();
     if (threadIdx.x == 0) s_last = atomicAdd(&p.counters[tile], 1) == p.splits - 1;
@@ -345,21 +348,21 @@ __global__ __launch_bounds__(kThreads) void split_gemm_kernel(SplitArgs p) {
     __threadfence();
     // acc := slot 0 + slot 1 + ... in split order (this split's own slot is
     // read back too, so the sum is the same whichever split finishes last).
-    // One block row (64 values) per pass, its loads issued together: a whole
-    // slot at once (256 values beside the 256 accumulators) does not fit in
-    // registers and would spill.
+    // One block row (32 values) per pass, its loads issued together: a whole
+    // slot at once (128 values beside the 128 accumulators) would crowd the
+    // registers.
     const float *base = p.partials + (int64_t)tile * p.splits * kSlot + threadIdx.x;
 #pragma unroll
     for (int i = 0; i < 4; ++i)
       for (int q = 0; q < p.splits; ++q) {
         const float *slot = base + (int64_t)q * kSlot;
-        float v[4][16];
+        float v[kFN][16];
 #pragma unroll
-        for (int j = 0; j < 4; ++j)
+        for (int j = 0; j < kFN; ++j)
 #pragma unroll
-          for (int r = 0; r < 16; ++r) v[j][r] = slot[((i * 4 + j) * 16 + r) * kThreads];
+          for (int r = 0; r < 16; ++r) v[j][r] = slot[((i * kFN + j) * 16 + r) * kThreads];
 #pragma unroll
-        for (int j = 0; j < 4; ++j)
+        for (int j = 0; j < kFN; ++j)
 #pragma unroll
           for (int r = 0; r < 16; ++r)
             g.acc[i][j][r] = q == 0 ? v[j][r] : g.acc[i][j][r] + v[j][r];
@@ -368,7 +371,7 @@ __global__ __launch_bounds__(kThreads) void split_gemm_kernel(SplitArgs p) {
   }
 
   // Element r of lane l in block (i, j): C row m0+wm+32i+l%32, column
-  // n0+wn+32j+8(r/4)+4(l/32)+r%4. The C loads of a block row (4 blocks) are
+  // n0+wn+32j+8(r/4)+4(l/32)+r%4. The C loads of a block row (2 blocks) are
   // issued together without branches (an element outside C reads C[0]
   // instead and is not stored): one memory round trip per block row.
   const bool read_c = p.beta != 0.f;  // uniform
@@ -376,10 +379,10 @@ __global__ __launch_bounds__(kThreads) void split_gemm_kernel(SplitArgs p) {
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
     const int gi = m0 + g.wm + 32 * i + r32;
-    float cv[4][16];
+    float cv[kFN][16];
     if (read_c) {
 #pragma unroll
-      for (int j = 0; j < 4; ++j)
+      for (int j = 0; j < kFN; ++j)
 #pragma unroll
         for (int r = 0; r < 16; ++r) {
           const int gj = n0 + g.wn + 32 * j + 8 * (r >> 2) + 4 * hi + (r & 3);
@@ -388,7 +391,7 @@ __global__ __launch_bounds__(kThreads) void split_gemm_kernel(SplitArgs p) {
         }
     }
 #pragma unroll
-    for (int j = 0; j < 4; ++j)
+    for (int j = 0; j < kFN; ++j)
 #pragma unroll
       for (int r = 0; r < 16; ++r) {
         const int gj = n0 + g.wn + 32 * j + 8 * (r >> 2) + 4 * hi + (r & 3);
