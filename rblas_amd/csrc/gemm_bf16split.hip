@@ -33,12 +33,17 @@ typedef __attribute__((address_space(3))) void *lds_void_ptr;
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
-constexpr int kWaves = 8;              // 2 x 4 over the C tile, two per SIMD
-constexpr int kThreads = 64 * kWaves;
 constexpr int kTile = 256;             // C tile rows and columns; plane row padding
 constexpr int kWave = 128;             // per-wave C tile rows: 4 blocks of 32
-constexpr int kWaveN = 64;             // per-wave C tile columns: 2 blocks of 32
-constexpr int kFN = kWaveN / 32;
+
+// W waves per workgroup, 2 x (W / 2) over the C tile: 8 (two per SIMD, each
+// 128 x 64) or 4 (one per SIMD, each 128 x 128). Both give the same bits.
+template <int W>
+struct Waves {
+  static constexpr int kThreads = 64 * W;
+  static constexpr int kWaveN = 2 * kTile / W;  // per-wave C tile columns
+  static constexpr int kFN = kWaveN / 32;       // 32-column blocks per wave
+};
 constexpr int kBK = 16;                // k per LDS stage (one MFMA k-step); plane k padding
 constexpr int kStages = 3;
 constexpr int kPlaneTileBytes = kTile * kBK * 2;  // 8 KB: 256 rows x 32 B
@@ -182,8 +187,9 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t block_rsrc(const char *base) {
   return __builtin_amdgcn_make_buffer_rsrc(const_cast<char *>(base), (short)0, 1024, 0x00020000);
 }
 
-template <int PL>
+template <int PL, int W>
 struct SplitGemm {
+  static constexpr int kWaves = W, kFN = Waves<W>::kFN;
   static constexpr int kPerWave = 8 * PL / kWaves;  // DMA blocks (1 KB = 32 rows) per operand per wave
   static constexpr int kPerTile = 2 * kPerWave;  // DMA instructions per wave per k-tile
   const char *ga[kPerWave];  // start of each of this wave's A blocks at k-block 0
@@ -282,10 +288,11 @@ struct SplitGemm {
 // itself on the tile's counter; the split that counts last (acquire) sums
 // the slots in split order -- a fixed order, so the bits do not depend on
 // which workgroup finishes last -- runs the epilogue and resets the counter.
-template <int PL>
-__global__ __launch_bounds__(kThreads) void split_gemm_kernel(SplitArgs p) {
+template <int PL, int W>
+__global__ __launch_bounds__(Waves<W>::kThreads) void split_gemm_kernel(SplitArgs p) {
+  constexpr int kThreads = Waves<W>::kThreads, kWaveN = Waves<W>::kWaveN, kFN = Waves<W>::kFN;
   __shared__ int s_last;
-  SplitGemm<PL> g;
+  SplitGemm<PL, W> g;
   const int tiles_m = (p.m + kTile - 1) / kTile, tiles_n = (p.n + kTile - 1) / kTile;
   const int tiles = tiles_m * tiles_n;
   int u = blockIdx.x, tm, tn;
@@ -494,10 +501,15 @@ hipError_t launch_sgemm_split(const GemmProblem<float> &p, int planes, void *wor
   SplitArgs a{wa, wb, pa, pb, (int)p.m, (int)p.n, (int)mp, (int)np, (int)kp, p.alpha, p.beta,
               p.c, p.ldc, sk.splits, sk.per, partials, counters};
   const dim3 grid((unsigned)(tiles * sk.splits));
-  if (planes == 2)
-    hipLaunchKernelGGL((split_gemm_kernel<2>), grid, dim3(kThreads), 0, stream, a);
+  // Two waves per SIMD hide the barrier and LDS waits of the square shapes;
+  // the k-split BF16x3 product (short per-split runs) measured faster with
+  // one wave per SIMD.
+  if (planes == 2 && sk.splits > 1)
+    hipLaunchKernelGGL((split_gemm_kernel<2, 4>), grid, dim3(Waves<4>::kThreads), 0, stream, a);
+  else if (planes == 2)
+    hipLaunchKernelGGL((split_gemm_kernel<2, 8>), grid, dim3(Waves<8>::kThreads), 0, stream, a);
   else
-    hipLaunchKernelGGL((split_gemm_kernel<3>), grid, dim3(kThreads), 0, stream, a);
+    hipLaunchKernelGGL((split_gemm_kernel<3, 8>), grid, dim3(Waves<8>::kThreads), 0, stream, a);
   return hipGetLastError();
 }
 
