@@ -1,6 +1,8 @@
 // Level-1 dot / nrm2 / axpy on gfx950. All three are HBM-bound: one pass
-// over the vectors with grid-stride loops (16-byte vector loads on the
-// contiguous, aligned path), wavefront reductions through lane shuffles, and
+// over the vectors (16-byte vector loads on the contiguous, aligned path:
+// for dot and axpy each workgroup streams one contiguous chunk, four
+// block-strides in flight; nrm2 and strided vectors use grid-stride loops),
+// wavefront reductions through lane shuffles, and
 // a second single-block kernel that folds the per-block partials in a fixed
 // order (results do not depend on scheduling).
 //
@@ -21,6 +23,7 @@ namespace {
 
 constexpr int kThreads = 256;
 constexpr int kMaxBlocks = 2048;
+constexpr int kUnroll = 4;  // block-strides per operand in flight per thread
 
 template <typename T>
 __device__ __forceinline__ T wave_sum(T v) {
@@ -52,6 +55,15 @@ struct Vec16<double> { typedef double type __attribute__((ext_vector_type(2))); 
 
 __device__ __forceinline__ bool aligned16(const void *p) { return ((uintptr_t)p & 15) == 0; }
 
+// This workgroup's contiguous share [c0, c1) of nv items: gridDim.x chunks
+// of whole block-strides (kThreads items), the last one shorter.
+__device__ __forceinline__ void chunk_of(int64_t nv, int64_t &c0, int64_t &c1) {
+  const int64_t strides = (nv + kThreads - 1) / kThreads;
+  const int64_t per = (strides + gridDim.x - 1) / gridDim.x * kThreads;
+  c0 = blockIdx.x * per;
+  c1 = c0 + per < nv ? c0 + per : nv;
+}
+
 // ------------------------------------------------------------------- dot
 template <typename T>
 __global__ __launch_bounds__(kThreads) void dot_partial(int64_t n, const T *x, int64_t incx,
@@ -65,12 +77,30 @@ __global__ __launch_bounds__(kThreads) void dot_partial(int64_t n, const T *x, i
     constexpr int W = Vec16<T>::n;
     const int64_t nv = n / W;
     const V *xv = (const V *)x, *yv = (const V *)y;
-    for (int64_t i = tid; i < nv; i += stride) {
+    // Each workgroup streams one contiguous chunk of the vectors, kUnroll
+    // block-strides (kUnroll x 4 KiB per operand) in flight per trip.
+    int64_t c0, c1;
+    chunk_of(nv, c0, c1);
+    int64_t i = c0 + threadIdx.x;
+    for (; i + (kUnroll - 1) * kThreads < c1; i += kUnroll * kThreads) {
+      V a[kUnroll], b[kUnroll];
+#pragma unroll
+      for (int u = 0; u < kUnroll; ++u) {
+        a[u] = xv[i + u * kThreads];
+        b[u] = yv[i + u * kThreads];
+      }
+#pragma unroll
+      for (int u = 0; u < kUnroll; ++u)
+#pragma unroll
+        for (int e = 0; e < W; ++e)
+          (e & 1 ? acc1 : acc0) = fma_t(a[u][e], b[u][e], e & 1 ? acc1 : acc0);
+    }
+    for (; i < c1; i += kThreads) {
       const V a = xv[i], b = yv[i];
 #pragma unroll
       for (int e = 0; e < W; ++e) (e & 1 ? acc1 : acc0) = fma_t(a[e], b[e], e & 1 ? acc1 : acc0);
     }
-    for (int64_t i = nv * W + tid; i < n; i += stride) acc0 = fma_t(x[i], y[i], acc0);
+    for (int64_t r = nv * W + tid; r < n; r += stride) acc0 = fma_t(x[r], y[r], acc0);
   } else {
     for (int64_t i = tid; i < n; i += stride) acc0 = fma_t(x[i * incx], y[i * incy], acc0);
   }
@@ -99,14 +129,31 @@ __global__ __launch_bounds__(kThreads) void axpy_kernel(int64_t n, T alpha, cons
     const int64_t nv = n / W;
     const V *xv = (const V *)x;
     V *yv = (V *)y;
-    for (int64_t i = tid; i < nv; i += stride) {
+    int64_t c0, c1;
+    chunk_of(nv, c0, c1);
+    int64_t i = c0 + threadIdx.x;
+    for (; i + (kUnroll - 1) * kThreads < c1; i += kUnroll * kThreads) {
+      V a[kUnroll], b[kUnroll];
+#pragma unroll
+      for (int u = 0; u < kUnroll; ++u) {
+        a[u] = xv[i + u * kThreads];
+        b[u] = yv[i + u * kThreads];
+      }
+#pragma unroll
+      for (int u = 0; u < kUnroll; ++u) {
+#pragma unroll
+        for (int e = 0; e < W; ++e) b[u][e] = fma_t(alpha, a[u][e], b[u][e]);
+        yv[i + u * kThreads] = b[u];
+      }
+    }
+    for (; i < c1; i += kThreads) {
       const V a = xv[i];
       V b = yv[i];
 #pragma unroll
       for (int e = 0; e < W; ++e) b[e] = fma_t(alpha, a[e], b[e]);
       yv[i] = b;
     }
-    for (int64_t i = nv * W + tid; i < n; i += stride) y[i] = fma_t(alpha, x[i], y[i]);
+    for (int64_t r = nv * W + tid; r < n; r += stride) y[r] = fma_t(alpha, x[r], y[r]);
   } else {
     for (int64_t i = tid; i < n; i += stride)
       y[i * incy] = fma_t(alpha, x[i * incx], y[i * incy]);
