@@ -36,7 +36,9 @@ synthetic data of the stated shape:
 `roofline` prices the DGEMM kernel against MI355X's FP64 MFMA peak, with HBM
 traffic per launch from the committed rocprofv3 PMC summary; `cpu_baseline`
 times the oracle (rblas's CPU algorithm restated in C++) on a column slice
-of the DGEMM problem on the host cores.
+of the DGEMM problem on the host cores, and `cpu_baseline.rows` the oracle
+on the grouped GEMM and batched trsm workloads, a prefix of the serving
+trace and the f64 Level-1 calls (full runs only).
 """
 from __future__ import annotations
 
@@ -277,21 +279,48 @@ def skinny_sgemm_tflops(steps=5):
     return out
 
 
-def grouped_gemm(dtype):
-    from rblas_amd import grouped, level3, workloads
-    from rblas_amd.device import DeviceArray, Stream
+def grouped_gemm_set(dtype):
+    """The grouped GEMM workload, one problem at a time: 512 column-major
+    problems, m, n, k = floor(exp U[ln 16, ln 2049]), transposes uniform over
+    the four combinations, alpha 1, beta 0.5, operands U[-1, 1] drawn from one
+    stream (seed 21) in problem order. Yields (ta, tb, m, n, k, lda, ldb,
+    [A, B, C0])."""
+    from rblas_amd import workloads
     rng = np.random.default_rng(4)
-    s = Stream()
-    probs, off, flops = [], 0, 0.0
+    off = 0
     for _ in range(512):
         m, n, k = (int(np.exp(rng.uniform(np.log(16), np.log(2049)))) for _ in range(3))
         ta, tb = int(rng.integers(0, 2)), int(rng.integers(0, 2))
         ar, ac = (k, m) if ta else (m, k)
         br, bc = (n, k) if tb else (k, n)
-        arrs = []
+        host = []
         for cnt in (ar * ac, br * bc, m * n):
-            arrs.append(DeviceArray.from_host(workloads.uniform(21, off, cnt, dtype), s))
+            host.append(workloads.uniform(21, off, cnt, dtype))
             off += cnt
+        yield ta, tb, m, n, k, ar, br, host
+
+
+def grouped_trsm_set():
+    """The batched trsm workload: 256 lower-triangular f64 systems (Left,
+    NoTrans, NonUnit, alpha 1), order floor(exp U[ln 32, ln 1025]), right-hand
+    sides U{1..256}, diagonal 2 + |N(0,1)|, off-diagonal N(0,1)/sqrt(order),
+    seed 5. Yields (order, nrhs, A, B0), A column-major."""
+    rng = np.random.default_rng(5)
+    for _ in range(256):
+        q = int(np.exp(rng.uniform(np.log(32), np.log(1025))))
+        nr = int(rng.integers(1, 257))
+        full = rng.standard_normal((q, q)) / np.sqrt(q)
+        np.fill_diagonal(full, 2.0 + np.abs(rng.standard_normal(q)))
+        yield q, nr, np.tril(full).T.reshape(-1).copy(), rng.standard_normal(q * nr)
+
+
+def grouped_gemm(dtype):
+    from rblas_amd import grouped, level3
+    from rblas_amd.device import DeviceArray, Stream
+    s = Stream()
+    probs, flops = [], 0.0
+    for ta, tb, m, n, k, ar, br, host in grouped_gemm_set(dtype):
+        arrs = [DeviceArray.from_host(h, s) for h in host]
         probs.append(grouped.GemmProblem(1, ta, tb, m, n, k, 1.0, arrs[0], ar, arrs[1], br, 0.5,
                                          arrs[2], m))
         flops += 2.0 * m * n * k
@@ -327,18 +356,13 @@ def grouped_gemm(dtype):
 
 
 def grouped_trsm():
-    from rblas_amd import grouped, level3, workloads
+    from rblas_amd import grouped, level3
     from rblas_amd.device import DeviceArray, Stream
-    rng = np.random.default_rng(5)
     s = Stream()
     probs, flops, ideal_s = [], 0.0, 0.0
-    for _ in range(256):
-        q = int(np.exp(rng.uniform(np.log(32), np.log(1025))))
-        nr = int(rng.integers(1, 257))
-        full = rng.standard_normal((q, q)) / np.sqrt(q)
-        np.fill_diagonal(full, 2.0 + np.abs(rng.standard_normal(q)))
-        a = DeviceArray.from_host(np.tril(full).T.reshape(-1).copy(), s)
-        b = DeviceArray.from_host(rng.standard_normal(q * nr), s)
+    for q, nr, ha, hb in grouped_trsm_set():
+        a = DeviceArray.from_host(ha, s)
+        b = DeviceArray.from_host(hb, s)
         probs.append(grouped.TrsmProblem(0, 1, 0, 0, q, nr, 1.0, a, q, b, q))
         flops += float(q) * q * nr
         # Per-system roofline: max(q^2 nrhs / FP64 peak, (q^2/2 + 2 q nrhs) 8 B / HBM).
@@ -488,6 +512,7 @@ def level2_bw(n=16384, steps=10):
 
 
 def serving(count):
+    """Returns (summary, the timed request trace)."""
     from rblas_amd import serve
     closed = serve.workload(count, 0.0)
     res, st = serve.run(closed, serve.BATCHED)
@@ -502,7 +527,7 @@ def serving(count):
     return {"requests": count, "rate_rps": round(rate, 1),
             "capacity_rps": round(capacity, 1),
             "batched": {k: round(batched[k], 3) for k in keys},
-            "plain": {k: round(plain[k], 3) for k in keys}}
+            "plain": {k: round(plain[k], 3) for k in keys}}, trace
 
 
 def c64_gemm(n=4096):
@@ -570,6 +595,82 @@ def cpu_baseline(n, ld):
                       f"median of 3, oracle restatement of rblas's threaded CPU gemm"}
 
 
+def cpu_rows(serve_trace, budget_s=10.0):
+    """The CPU side of the batching, serving and Level-1 rows, on the same
+    synthetic inputs as their GPU figures, timed with the oracle (the
+    restatement of rblas's CPU algorithms): gemm threaded as rblas's
+    `threads` feature is, trsm and Level-1 serial as rblas's are.
+
+    - grouped GEMM (f64, f32): the 512 problems one after another;
+    - batched trsm: the 256 systems, one serial solve per system, spread over
+      the worker threads (the systems are independent);
+    - serving: the serving trace's requests in arrival order on one FIFO
+      server, service times measured and operand generation not timed,
+      until 100 requests or `budget_s` of service time;
+    - Level-1: dot / axpy / nrm2 at n = 2^26, f64, one core."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    from oracle import oracle as orc
+    from rblas_amd import serve, workloads
+    threads = min(os.cpu_count() or 1, 16)
+    out = {"cores": threads, "kind": "port"}
+    for dtype, name in ((np.float64, "grouped_gemm_f64"), (np.float32, "grouped_gemm_f32")):
+        flops, secs = 0.0, 0.0
+        for ta, tb, m, n, k, ar, br, (a, b, c) in grouped_gemm_set(dtype):
+            t = time.perf_counter()
+            orc.gemm(1, ta, tb, m, n, k, 1.0, a, ar, b, br, 0.5, c, m, threads=threads)
+            secs += time.perf_counter() - t
+            flops += 2.0 * m * n * k
+        out[name] = {"tflops": round(flops / secs / 1e12, 4), "ms": round(secs * 1e3, 1)}
+    systems = list(grouped_trsm_set())
+    flops = sum(float(q) * q * nr for q, nr, _, _ in systems)
+    t = time.perf_counter()
+    with ThreadPoolExecutor(threads) as pool:
+        list(pool.map(lambda p: orc.trsm(0, 1, 0, 0, p[0], p[1], 1.0, p[2], p[0], p[3], p[0]),
+                      systems))
+    secs = time.perf_counter() - t
+    del systems
+    out["grouped_trsm_f64"] = {"tflops": round(flops / secs / 1e12, 5), "ms": round(secs * 1e3, 1)}
+    free_at, service, flops, lat = 0.0, 0.0, 0.0, []
+    for req in serve_trace[:100]:
+        if service >= budget_s:
+            break
+        kind, m, n, k = (int(req[f]) for f in ("kind", "m", "n", "k"))
+        ops = serve.operands(req)
+        t = time.perf_counter()
+        if kind == serve.GEMM:
+            orc.gemm(1, 0, 0, m, n, k, 1.0, ops[0], m, ops[1], k, 0.5, ops[2], m,
+                     threads=threads)
+            f = 2.0 * m * n * k
+        elif kind == serve.TRSM:
+            orc.trsm(0, 1, 0, 0, m, n, 1.0, ops[0], m, ops[1], m)
+            f = float(m) * m * n
+        else:
+            orc.cgemm(0, 0, m, n, k, 1.0, ops[0], m, ops[1], k, 0.5, ops[2], m, threads=threads)
+            f = 8.0 * m * n * k
+        d = time.perf_counter() - t
+        start = max(float(req["arrival_s"]), free_at)
+        free_at = start + d
+        lat.append(free_at - float(req["arrival_s"]))
+        service += d
+        flops += f
+    out["serve_prefix"] = {"requests": len(lat), "tflops": round(flops / service / 1e12, 4),
+                           "p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 1),
+                           "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 1),
+                           "service_s": round(service, 2)}
+    n = 1 << 26
+    x = workloads.uniform(3, 0, n, np.float64)
+    y = workloads.uniform(3, n, n, np.float64)
+    for name, fn, nbytes in (("ddot", lambda: orc.dot(n, x, 1, y, 1), 16 * n),
+                             ("daxpy", lambda: orc.axpy(n, 1e-3, x, 1, y, 1), 24 * n),
+                             ("dnrm2", lambda: orc.nrm2(n, x, 1), 8 * n)):
+        fn()
+        t = time.perf_counter()
+        fn()
+        out[name + "_gbs_1core"] = round(nbytes / (time.perf_counter() - t) / 1e9, 2)
+    return out
+
+
 ONLY = {
     "sgemm_bf16x3": lambda a: {"ms": square_gemm(np.float32, a.size, a.size + 7, 5, 2,
                                                  compute=1)},
@@ -583,7 +684,7 @@ ONLY = {
     "symm_hemm": lambda a: symm_hemm(),
     "ctrsm": lambda a: ctrsm_single(),
     "level2": lambda a: level2_bw(),
-    "serve": lambda a: serving(a.serve_requests),
+    "serve": lambda a: serving(a.serve_requests)[0],
     "c64_gemm": lambda a: c64_gemm(),
     "sgemm_skinny": lambda a: skinny_sgemm_tflops(),
     "level1": lambda a: level1_bw(),
@@ -671,7 +772,7 @@ def main():
         log("grouped trsm ...")
         comp["grouped_trsm_f64"] = grouped_trsm()
         log("serving ...")
-        comp["serve"] = serving(args.serve_requests)
+        comp["serve"], trace = serving(args.serve_requests)
         comp["serve_p99_ms"] = comp["serve"]["batched"]["p99_ms"]
         log("c64 / level-1 ...")
         comp["c64_gemm_4096"] = c64_gemm()
@@ -682,6 +783,9 @@ def main():
     if not args.no_cpu_baseline:
         log("cpu baseline ...")
         out["cpu_baseline"] = cpu_baseline(n, ld)
+        if not args.quick:
+            log("cpu rows ...")
+            out["cpu_baseline"]["rows"] = cpu_rows(trace)
     print(json.dumps(out), flush=True)
 
 
