@@ -34,24 +34,30 @@ __device__ __forceinline__ T op_a(const T *a, int64_t lda, bool trans, int64_t r
 }
 
 // Left leaf: B[r0 + i, col] := sum_{j in tri(i)} inv[i][j] * B[r0 + j, col]
-// for i, j < nb and the 64 columns of this workgroup. 256 threads: lane row
-// i = t % 64 (coalesced on column-major B), columns t / 64 + 4 r.
+// for i, j < nb and the kLeafCols columns of this workgroup. 256 threads:
+// lane row i = t % 64 (coalesced on column-major B), columns t / 64 + 4 r.
+// 32 columns per workgroup put 256 workgroups on a 64 x 8192 leaf (64 columns
+// gave 128, half the CUs, and a 16-column FMA chain per lane; 16 columns
+// measured 1% slower in the leaf pair, whose workgroups each re-read two
+// inverses and a block of op(A)).
+constexpr int kLeafCols = 32;
 template <typename T>
 __global__ __launch_bounds__(256) void trsm_apply_left(const double *inv, bool upper, int nb,
                                                        int64_t n, T *b, int64_t ldb) {
-  __shared__ double invT[kNB][kNB];  // invT[j][i] = inv(i, j)
-  __shared__ double Bs[kNB][kNB];    // Bs[c][j] = B(r0 + j, col0 + c)
+  constexpr int R = kLeafCols / 4;  // columns per lane
+  __shared__ double invT[kNB][kNB];     // invT[j][i] = inv(i, j)
+  __shared__ double Bs[kLeafCols][kNB];  // Bs[c][j] = B(r0 + j, col0 + c)
   const int t = threadIdx.x, i = t % kNB, c0 = t / kNB;
-  const int64_t col0 = (int64_t)blockIdx.x * kNB;
-  const int ncol = (int)((n - col0) < kNB ? (n - col0) : kNB);
+  const int64_t col0 = (int64_t)blockIdx.x * kLeafCols;
+  const int ncol = (int)((n - col0) < kLeafCols ? (n - col0) : kLeafCols);
   // Every load issued before any is used, without branches (an entry outside
   // the block reads b[0] and is replaced by 0).
   double iv[16];
-  T raw[16];
+  T raw[R];
 #pragma unroll
   for (int r = 0; r < 16; ++r) iv[r] = inv[t + 256 * r];
 #pragma unroll
-  for (int r = 0; r < 16; ++r) {
+  for (int r = 0; r < R; ++r) {
     const int c = c0 + 4 * r;
     const bool ok = c < ncol && i < nb;
     raw[r] = *(ok ? b + i + (col0 + c) * ldb : b);
@@ -60,13 +66,16 @@ __global__ __launch_bounds__(256) void trsm_apply_left(const double *inv, bool u
   for (int r = 0; r < 16; ++r) {
     const int idx = t + 256 * r;
     invT[idx / kNB][idx % kNB] = iv[r];
+  }
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
     const int c = c0 + 4 * r;
     Bs[c][i] = (c < ncol && i < nb) ? (double)raw[r] : 0.0;
   }
   __syncthreads();
-  double acc[16];
+  double acc[R];
 #pragma unroll
-  for (int r = 0; r < 16; ++r) acc[r] = 0.0;
+  for (int r = 0; r < R; ++r) acc[r] = 0.0;
   // Row i sums over its triangle only: j = 0..i (lower) or nb-1 down to i
   // (upper). j is the same in every lane still running, so Bs reads
   // broadcast; lanes leave the loop when their triangle ends.
@@ -75,38 +84,163 @@ __global__ __launch_bounds__(256) void trsm_apply_left(const double *inv, bool u
     const int j = upper ? nb - 1 - s : s;
     const double w = invT[j][i];
 #pragma unroll
-    for (int r = 0; r < 16; ++r) acc[r] = __builtin_fma(w, Bs[c0 + 4 * r][j], acc[r]);
+    for (int r = 0; r < R; ++r) acc[r] = __builtin_fma(w, Bs[c0 + 4 * r][j], acc[r]);
   }
   if (i < nb) {
 #pragma unroll
-    for (int r = 0; r < 16; ++r) {
+    for (int r = 0; r < R; ++r) {
       const int c = c0 + 4 * r;
       if (c < ncol) b[i + (col0 + c) * ldb] = (T)acc[r];
     }
   }
 }
 
+// Left leaf pair: the recursion's last level in one launch. A diagonal
+// range of 65..128 rows splits as the solver splits it, first f_len rows at
+// f_off then s_len rows at s_off (one of them a whole 64-row block); per
+// kLeafCols columns: X_f = inv_f B_f (triangle only, as trsm_apply_left);
+// B_s -= op(A)[s, f] X_f (a_sf points at op(A)(s_lo, f_lo)); X_s = inv_s B_s.
+// It replaces a leaf, a k = 64 gemm update of 64 rows and a second leaf.
+template <typename T>
+__global__ __launch_bounds__(256) void trsm_apply_left_pair(const double *inv_f,
+                                                            const double *inv_s, const T *a_sf,
+                                                            int64_t lda, bool trans, bool upper,
+                                                            int f_off, int f_len, int s_off,
+                                                            int s_len, int64_t n, T *b,
+                                                            int64_t ldb) {
+  constexpr int R = kLeafCols / 4;  // columns per lane
+  __shared__ double W[kNB][kNB];              // W[j][i]: inv_f(i, j), op(A)(s+i, f+j), inv_s(i, j)
+  __shared__ double Bs[kLeafCols][2 * kNB];   // Bs[c][r] = B(lo + r, col0 + c)
+  const int t = threadIdx.x, i = t % kNB, c0 = t / kNB;
+  const int len = f_len + s_len;
+  const int64_t col0 = (int64_t)blockIdx.x * kLeafCols;
+  const int ncol = (int)((n - col0) < kLeafCols ? (n - col0) : kLeafCols);
+  // Every global load issued before any is used; an entry outside its block
+  // reads element 0 and is replaced by 0.
+  double ivf[16], ivs[16];
+  T av[16], raw[2][R];
+#pragma unroll
+  for (int r = 0; r < 16; ++r) ivf[r] = inv_f[t + 256 * r];
+#pragma unroll
+  for (int h = 0; h < 2; ++h)
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int row = h * kNB + i, c = c0 + 4 * r;
+      raw[h][r] = *(row < len && c < ncol ? b + row + (col0 + c) * ldb : b);
+    }
+#pragma unroll
+  for (int r = 0; r < 16; ++r) {
+    const int idx = t + 256 * r, ai = idx % kNB, aj = idx / kNB;
+    const bool ok = ai < s_len && aj < f_len;
+    av[r] = *(ok ? (trans ? a_sf + aj + ai * lda : a_sf + ai + aj * lda) : a_sf);
+    ivs[r] = inv_s[t + 256 * r];
+  }
+#pragma unroll
+  for (int r = 0; r < 16; ++r) {
+    const int idx = t + 256 * r;
+    W[idx / kNB][idx % kNB] = ivf[r];
+  }
+#pragma unroll
+  for (int h = 0; h < 2; ++h)
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int row = h * kNB + i, c = c0 + 4 * r;
+      Bs[c][row] = (row < len && c < ncol) ? (double)raw[h][r] : 0.0;
+    }
+  __syncthreads();
+  double acc[R];
+  // X_f: row i of the f block sums over its triangle (as trsm_apply_left).
+#pragma unroll
+  for (int r = 0; r < R; ++r) acc[r] = 0.0;
+  int cnt = i < f_len ? (upper ? f_len - i : i + 1) : 0;
+  for (int s = 0; s < cnt; ++s) {
+    const int j = upper ? f_len - 1 - s : s;
+    const double w = W[j][i];
+#pragma unroll
+    for (int r = 0; r < R; ++r) acc[r] = __builtin_fma(w, Bs[c0 + 4 * r][f_off + j], acc[r]);
+  }
+  __syncthreads();  // every lane has read W and the f rows
+  if (i < f_len) {
+#pragma unroll
+    for (int r = 0; r < R; ++r) Bs[c0 + 4 * r][f_off + i] = acc[r];
+  }
+#pragma unroll
+  for (int r = 0; r < 16; ++r) {
+    const int idx = t + 256 * r, ai = idx % kNB, aj = idx / kNB;
+    W[aj][ai] = (ai < s_len && aj < f_len) ? (double)av[r] : 0.0;
+  }
+  __syncthreads();
+  // B_s -= op(A)[s, f] X_f: lanes read only f rows and write only their own
+  // s row.
+  if (i < s_len) {
+#pragma unroll
+    for (int r = 0; r < R; ++r) acc[r] = 0.0;
+    for (int j = 0; j < f_len; ++j) {
+      const double w = W[j][i];
+#pragma unroll
+      for (int r = 0; r < R; ++r) acc[r] = __builtin_fma(w, Bs[c0 + 4 * r][f_off + j], acc[r]);
+    }
+#pragma unroll
+    for (int r = 0; r < R; ++r) Bs[c0 + 4 * r][s_off + i] -= acc[r];
+  }
+  __syncthreads();  // every lane has read W; the s rows are updated
+#pragma unroll
+  for (int r = 0; r < 16; ++r) {
+    const int idx = t + 256 * r;
+    W[idx / kNB][idx % kNB] = ivs[r];
+  }
+  __syncthreads();
+  // X_s over the s block's triangle.
+#pragma unroll
+  for (int r = 0; r < R; ++r) acc[r] = 0.0;
+  cnt = i < s_len ? (upper ? s_len - i : i + 1) : 0;
+  for (int s = 0; s < cnt; ++s) {
+    const int j = upper ? s_len - 1 - s : s;
+    const double w = W[j][i];
+#pragma unroll
+    for (int r = 0; r < R; ++r) acc[r] = __builtin_fma(w, Bs[c0 + 4 * r][s_off + j], acc[r]);
+  }
+  if (i < s_len) {
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int c = c0 + 4 * r;
+      if (c < ncol) b[s_off + i + (col0 + c) * ldb] = (T)acc[r];
+    }
+  }
+  if (i < f_len) {
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int c = c0 + 4 * r;
+      if (c < ncol) b[f_off + i + (col0 + c) * ldb] = (T)Bs[c][f_off + i];
+    }
+  }
+}
+
 // Right leaf: B[row, r0 + j] := sum_{p in tri(j)} B[row, r0 + p] * inv[p][j]
-// for the 64 rows of this workgroup. Loads and stores use lane = row
-// (coalesced on column-major B); the sums use lane = column j = t % 64 and
-// rows t / 64 + 4 r, so each lane's triangle (p = 0..j upper, nb-1 down to j
-// lower) is its own loop trip count, p is the same in every lane still
-// running and the row reads broadcast.
+// for the kLeafRows rows of this workgroup. Loads and stores use lane = row
+// t % kLeafRows (runs of 16 rows of column-major B); the sums use lane =
+// column j = t % 64 and rows t / 64 + 4 r, so each lane's triangle (p = 0..j
+// upper, nb-1 down to j lower) is its own loop trip count, p is the same in
+// every lane still running and the row reads broadcast. 16 rows per
+// workgroup put 512 workgroups on an 8192 x 64 leaf.
+constexpr int kLeafRows = 16;
 template <typename T>
 __global__ __launch_bounds__(256) void trsm_apply_right(const double *inv, bool upper, int nb,
                                                         int64_t m, T *b, int64_t ldb) {
-  __shared__ double inv_t[kNB][kNB + 1];  // inv_t[p][j] = inv(p, j)
-  __shared__ double Bs[kNB][kNB + 1];     // Bs[p][i] = B(row0 + i, r0 + p)
-  const int t = threadIdx.x, i = t % kNB, c0 = t / kNB;
-  const int64_t row0 = (int64_t)blockIdx.x * kNB;
+  constexpr int R = kLeafRows / 4;       // rows per lane in the sums
+  constexpr int L = kNB * kLeafRows / 256;  // B entries per lane in loads / stores
+  __shared__ double inv_t[kNB][kNB + 1];        // inv_t[p][j] = inv(p, j)
+  __shared__ double Bs[kNB][kLeafRows + 1];     // Bs[p][i] = B(row0 + i, r0 + p)
+  const int t = threadIdx.x, i = t % kLeafRows, c0 = t / kLeafRows;
+  const int64_t row0 = (int64_t)blockIdx.x * kLeafRows;
   const bool row_ok = row0 + i < m;
   double iv[16];
-  T raw[16];
+  T raw[L];
 #pragma unroll
   for (int r = 0; r < 16; ++r) iv[r] = inv[t + 256 * r];
 #pragma unroll
-  for (int r = 0; r < 16; ++r) {
-    const int p = c0 + 4 * r;
+  for (int r = 0; r < L; ++r) {
+    const int p = c0 + (256 / kLeafRows) * r;
     const bool ok = row_ok && p < nb;
     raw[r] = *(ok ? b + row0 + i + p * ldb : b);
   }
@@ -114,29 +248,32 @@ __global__ __launch_bounds__(256) void trsm_apply_right(const double *inv, bool 
   for (int r = 0; r < 16; ++r) {
     const int idx = t + 256 * r;
     inv_t[idx % kNB][idx / kNB] = iv[r];
-    const int p = c0 + 4 * r;
+  }
+#pragma unroll
+  for (int r = 0; r < L; ++r) {
+    const int p = c0 + (256 / kLeafRows) * r;
     Bs[p][i] = (row_ok && p < nb) ? (double)raw[r] : 0.0;
   }
   __syncthreads();
   const int j = t % kNB, rw = t / kNB;  // column j; rows rw + 4 r
-  double acc[16];
+  double acc[R];
 #pragma unroll
-  for (int r = 0; r < 16; ++r) acc[r] = 0.0;
+  for (int r = 0; r < R; ++r) acc[r] = 0.0;
   const int cnt = j < nb ? (upper ? j + 1 : nb - j) : 0;
   for (int s = 0; s < cnt; ++s) {
     const int p = upper ? s : nb - 1 - s;
     const double w = inv_t[p][j];
 #pragma unroll
-    for (int r = 0; r < 16; ++r) acc[r] = __builtin_fma(Bs[p][rw + 4 * r], w, acc[r]);
+    for (int r = 0; r < R; ++r) acc[r] = __builtin_fma(Bs[p][rw + 4 * r], w, acc[r]);
   }
   __syncthreads();  // every lane has read Bs
 #pragma unroll
-  for (int r = 0; r < 16; ++r) Bs[j][rw + 4 * r] = acc[r];
+  for (int r = 0; r < R; ++r) Bs[j][rw + 4 * r] = acc[r];
   __syncthreads();
   if (row_ok) {
 #pragma unroll
-    for (int r = 0; r < 16; ++r) {
-      const int p = c0 + 4 * r;
+    for (int r = 0; r < L; ++r) {
+      const int p = c0 + (256 / kLeafRows) * r;
       if (p < nb) b[row0 + i + p * ldb] = (T)Bs[p][i];
     }
   }
@@ -179,12 +316,13 @@ struct Solver {
     if (len <= kNB) {
       const double *ib = inv + (lo / kNB) * kNB * kNB;
       if (p.left)
-        hipLaunchKernelGGL((trsm_apply_left<T>), dim3((unsigned)((p.n + kNB - 1) / kNB)),
-                           dim3(256), 0, st, ib, p.upper, (int)len, p.n, p.b + lo, p.ldb);
+        hipLaunchKernelGGL((trsm_apply_left<T>),
+                           dim3((unsigned)((p.n + kLeafCols - 1) / kLeafCols)), dim3(256), 0,
+                           st, ib, p.upper, (int)len, p.n, p.b + lo, p.ldb);
       else
-        hipLaunchKernelGGL((trsm_apply_right<T>), dim3((unsigned)((p.m + kNB - 1) / kNB)),
-                           dim3(256), 0, st, ib, p.upper, (int)len, p.m, p.b + lo * p.ldb,
-                           p.ldb);
+        hipLaunchKernelGGL((trsm_apply_right<T>),
+                           dim3((unsigned)((p.m + kLeafRows - 1) / kLeafRows)), dim3(256), 0,
+                           st, ib, p.upper, (int)len, p.m, p.b + lo * p.ldb, p.ldb);
       return hipGetLastError();
     }
     const int64_t h = ((len / 2 + kNB - 1) / kNB) * kNB;
@@ -194,6 +332,15 @@ struct Solver {
       f_len = len - h;
       s_lo = lo;
       s_len = h;
+    }
+    if (p.left && len <= 2 * kNB) {
+      hipLaunchKernelGGL((trsm_apply_left_pair<T>),
+                         dim3((unsigned)((p.n + kLeafCols - 1) / kLeafCols)), dim3(256), 0, st,
+                         inv + (f_lo / kNB) * kNB * kNB, inv + (s_lo / kNB) * kNB * kNB,
+                         sub(p.a, p.lda, p.trans, s_lo, f_lo), p.lda, p.trans, p.upper,
+                         (int)(f_lo - lo), (int)f_len, (int)(s_lo - lo), (int)s_len, p.n,
+                         p.b + lo, p.ldb);
+      return hipGetLastError();
     }
     hipError_t e = range(f_lo, f_len);
     if (e != hipSuccess) return e;

@@ -353,10 +353,12 @@ def test_trsm_device_entry_and_large(gpu, side):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("q", [1024, 1025, 2048, 2049, 2600])
-@pytest.mark.parametrize("side,uplo,trans", [(0, 1, 0), (0, 0, 1), (1, 0, 0), (1, 1, 1)])
+@pytest.mark.parametrize("side,uplo,trans", [(0, 1, 0), (0, 0, 1), (0, 0, 0), (0, 1, 1),
+                                             (1, 0, 0), (1, 1, 1), (1, 1, 0), (1, 0, 1)])
 def test_trsm_order_switches(gpu, q, side, uplo, trans):
     """Orders on both sides of the 16/8 right-hand-side width switch (1024)
-    and of the LDS-resident/recursive switch (2048), f32 and f64."""
+    and of the LDS-resident/recursive switch (2048), f32 and f64, forward
+    (op(A) lower on the Left) and backward substitution on both sides."""
     nrhs = 37
     m, n = (q, nrhs) if side == 0 else (nrhs, q)
     for dtype in (np.float32, np.float64):
