@@ -17,7 +17,7 @@ synthetic data of the stated shape:
     transposes, U[-1,1], f32 and f64) in one persistent launch, against the
     same problems as one launch each;
   - single trsm calls (Left / Lower / NoTrans, q = nrhs in {1024, 2048,
-    8192}, f64 and f32);
+    8192}, f64 and f32; Right / Lower / NoTrans at 8192, f64);
   - DSYMM 8192^3 and ZHEMM 4096^3 (dense A from the stored triangle, then
     the gemm);
   - grouped trsm (256 lower systems, n log-uniform [32, 1024], nrhs in
@@ -416,7 +416,8 @@ def symm_hemm():
 
 def trsm_single():
     """Single trsm calls on resident operands: Left / Lower / NoTrans,
-    q = nrhs in {1024, 2048, 8192}, f64 and f32 (TFLOP/s in q^2 nrhs)."""
+    q = nrhs in {1024, 2048, 8192}, f64 and f32, and Right / Lower / NoTrans
+    at 8192 f64 (TFLOP/s in q^3)."""
     from rblas_amd import level3
     from rblas_amd.device import DeviceArray, Stream
     rng = np.random.default_rng(11)
@@ -434,6 +435,14 @@ def trsm_single():
             steps = 5 if q == 8192 else 10
             ms = timed(lambda: level3.trsm_dev(0, 1, 0, 0, q, q, 1.0, da, q, db, q, s), s, steps, 2)
             out[f"{tag}_{q}"] = {"ms": round(ms, 3), "tflops": round(float(q) ** 3 / ms / 1e9, 2)}
+            da.free()
+            db.free()
+        if q == 8192:
+            da = DeviceArray.from_host(a64, s)
+            db = DeviceArray.from_host(b64, s)
+            ms = timed(lambda: level3.trsm_dev(1, 1, 0, 0, q, q, 1.0, da, q, db, q, s), s, 5, 2)
+            out["f64_8192_right"] = {"ms": round(ms, 3),
+                                     "tflops": round(float(q) ** 3 / ms / 1e9, 2)}
             da.free()
             db.free()
     s.close()
