@@ -279,6 +279,123 @@ __global__ __launch_bounds__(256) void trsm_apply_right(const double *inv, bool 
   }
 }
 
+// Right leaf pair: the recursion's last level on the Right in one launch.
+// The diagonal range of 65..128 columns splits as the solver splits it
+// (first f_len columns at f_off, then s_len at s_off); per kLeafRows rows of
+// B: X_f = B_f inv_f (as trsm_apply_right), B_s -= X_f op(A)[f, s] (a_fs
+// points at op(A)(f_lo, s_lo)), X_s = B_s inv_s.
+template <typename T>
+__global__ __launch_bounds__(256) void trsm_apply_right_pair(const double *inv_f,
+                                                             const double *inv_s, const T *a_fs,
+                                                             int64_t lda, bool trans, bool upper,
+                                                             int f_off, int f_len, int s_off,
+                                                             int s_len, int64_t m, T *b,
+                                                             int64_t ldb) {
+  constexpr int R = kLeafRows / 4;               // rows per lane in the sums
+  constexpr int L = 2 * kNB * kLeafRows / 256;   // B entries per lane in loads / stores
+  __shared__ double W[kNB][kNB + 1];             // W[p][j]: inv_f(p, j), op(A)(f+p, s+j), inv_s(p, j)
+  __shared__ double Bs[2 * kNB][kLeafRows + 1];  // Bs[p][i] = B(row0 + i, lo + p)
+  const int t = threadIdx.x, i = t % kLeafRows, c0 = t / kLeafRows;
+  const int len = f_len + s_len;
+  const int64_t row0 = (int64_t)blockIdx.x * kLeafRows;
+  const bool row_ok = row0 + i < m;
+  // Every global load issued before any is used; an entry outside its block
+  // reads element 0 and is replaced by 0.
+  double ivf[16], ivs[16];
+  T av[16], raw[L];
+#pragma unroll
+  for (int r = 0; r < 16; ++r) ivf[r] = inv_f[t + 256 * r];
+#pragma unroll
+  for (int r = 0; r < L; ++r) {
+    const int p = c0 + (256 / kLeafRows) * r;
+    raw[r] = *(row_ok && p < len ? b + row0 + i + p * ldb : b);
+  }
+#pragma unroll
+  for (int r = 0; r < 16; ++r) {
+    const int idx = t + 256 * r, ap = idx % kNB, aj = idx / kNB;
+    const bool ok = ap < f_len && aj < s_len;
+    av[r] = *(ok ? (trans ? a_fs + aj + ap * lda : a_fs + ap + aj * lda) : a_fs);
+    ivs[r] = inv_s[t + 256 * r];
+  }
+#pragma unroll
+  for (int r = 0; r < 16; ++r) {
+    const int idx = t + 256 * r;
+    W[idx % kNB][idx / kNB] = ivf[r];
+  }
+#pragma unroll
+  for (int r = 0; r < L; ++r) {
+    const int p = c0 + (256 / kLeafRows) * r;
+    Bs[p][i] = (row_ok && p < len) ? (double)raw[r] : 0.0;
+  }
+  __syncthreads();
+  const int j = t % kNB, rw = t / kNB;  // column j; rows rw + 4 r
+  double acc[R];
+  // X_f: column j of the f block sums over its triangle (as trsm_apply_right).
+#pragma unroll
+  for (int r = 0; r < R; ++r) acc[r] = 0.0;
+  int cnt = j < f_len ? (upper ? j + 1 : f_len - j) : 0;
+  for (int s = 0; s < cnt; ++s) {
+    const int q = upper ? s : f_len - 1 - s;
+    const double w = W[q][j];
+#pragma unroll
+    for (int r = 0; r < R; ++r) acc[r] = __builtin_fma(Bs[f_off + q][rw + 4 * r], w, acc[r]);
+  }
+  __syncthreads();  // every lane has read W and the f columns
+  if (j < f_len) {
+#pragma unroll
+    for (int r = 0; r < R; ++r) Bs[f_off + j][rw + 4 * r] = acc[r];
+  }
+#pragma unroll
+  for (int r = 0; r < 16; ++r) {
+    const int idx = t + 256 * r, ap = idx % kNB, aj = idx / kNB;
+    W[ap][aj] = (ap < f_len && aj < s_len) ? (double)av[r] : 0.0;
+  }
+  __syncthreads();
+  // B_s -= X_f op(A)[f, s]: lanes read only f columns and write only their
+  // own entries of s column j.
+  if (j < s_len) {
+#pragma unroll
+    for (int r = 0; r < R; ++r) acc[r] = 0.0;
+    for (int q = 0; q < f_len; ++q) {
+      const double w = W[q][j];
+#pragma unroll
+      for (int r = 0; r < R; ++r) acc[r] = __builtin_fma(Bs[f_off + q][rw + 4 * r], w, acc[r]);
+    }
+#pragma unroll
+    for (int r = 0; r < R; ++r) Bs[s_off + j][rw + 4 * r] -= acc[r];
+  }
+  __syncthreads();  // every lane has read W; the s columns are updated
+#pragma unroll
+  for (int r = 0; r < 16; ++r) {
+    const int idx = t + 256 * r;
+    W[idx % kNB][idx / kNB] = ivs[r];
+  }
+  __syncthreads();
+  // X_s over the s block's triangle.
+#pragma unroll
+  for (int r = 0; r < R; ++r) acc[r] = 0.0;
+  cnt = j < s_len ? (upper ? j + 1 : s_len - j) : 0;
+  for (int s = 0; s < cnt; ++s) {
+    const int q = upper ? s : s_len - 1 - s;
+    const double w = W[q][j];
+#pragma unroll
+    for (int r = 0; r < R; ++r) acc[r] = __builtin_fma(Bs[s_off + q][rw + 4 * r], w, acc[r]);
+  }
+  __syncthreads();  // every lane has read the s columns
+  if (j < s_len) {
+#pragma unroll
+    for (int r = 0; r < R; ++r) Bs[s_off + j][rw + 4 * r] = acc[r];
+  }
+  __syncthreads();
+  if (row_ok) {
+#pragma unroll
+    for (int r = 0; r < L; ++r) {
+      const int p = c0 + (256 / kLeafRows) * r;
+      if (p < len) b[row0 + i + p * ldb] = (T)Bs[p][i];
+    }
+  }
+}
+
 // B := alpha * B over m x n.
 template <typename T>
 __global__ void mul_kernel(int64_t m, int64_t n, T alpha, T *b, int64_t ldb) {
@@ -340,6 +457,15 @@ struct Solver {
                          sub(p.a, p.lda, p.trans, s_lo, f_lo), p.lda, p.trans, p.upper,
                          (int)(f_lo - lo), (int)f_len, (int)(s_lo - lo), (int)s_len, p.n,
                          p.b + lo, p.ldb);
+      return hipGetLastError();
+    }
+    if (!p.left && len <= 2 * kNB) {
+      hipLaunchKernelGGL((trsm_apply_right_pair<T>),
+                         dim3((unsigned)((p.m + kLeafRows - 1) / kLeafRows)), dim3(256), 0, st,
+                         inv + (f_lo / kNB) * kNB * kNB, inv + (s_lo / kNB) * kNB * kNB,
+                         sub(p.a, p.lda, p.trans, f_lo, s_lo), p.lda, p.trans, p.upper,
+                         (int)(f_lo - lo), (int)f_len, (int)(s_lo - lo), (int)s_len, p.m,
+                         p.b + lo * p.ldb, p.ldb);
       return hipGetLastError();
     }
     hipError_t e = range(f_lo, f_len);
