@@ -21,7 +21,8 @@ def oracle():
 
 @pytest.fixture(scope="session")
 def gpu():
-    """Skip-free GPU gate: on a GPU box the library must load and see a device."""
+    """Skip-free GPU gate: where the gpu tests run, the library must load and
+    see a device."""
     from rblas_amd import device
     n = device.device_count()
     assert n >= 1, "gpu tests need a visible HIP device"
