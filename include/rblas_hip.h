@@ -574,7 +574,9 @@ int32_t rblas_dtrsm_grouped_create(rblas_grouped_t **plan,
                                    const rblas_dtrsm_problem_t *problems,
                                    size_t count, size_t *bad_index);
 
-/* Enqueue one run on `stream` (not synchronised). */
+/* Enqueue one run on `stream` (not synchronised). Runs of one plan must not
+ * overlap: they share the plan's work counters (and write the same outputs),
+ * so enqueue them on one stream; different plans may run concurrently. */
 int32_t rblas_grouped_run(rblas_grouped_t *plan, void *stream);
 /* The same run, captured into a hipGraph on first use and replayed after;
  * needs a non-null stream. */
