@@ -218,12 +218,14 @@ __global__ __launch_bounds__(256) void trsm_apply_left_pair(const double *inv_f,
 
 // Right leaf: B[row, r0 + j] := sum_{p in tri(j)} B[row, r0 + p] * inv[p][j]
 // for the kLeafRows rows of this workgroup. Loads and stores use lane = row
-// t % kLeafRows (runs of 16 rows of column-major B); the sums use lane =
+// t % kLeafRows (runs of 32 rows of column-major B); the sums use lane =
 // column j = t % 64 and rows t / 64 + 4 r, so each lane's triangle (p = 0..j
 // upper, nb-1 down to j lower) is its own loop trip count, p is the same in
-// every lane still running and the row reads broadcast. 16 rows per
-// workgroup put 512 workgroups on an 8192 x 64 leaf.
-constexpr int kLeafRows = 16;
+// every lane still running and the row reads broadcast. 32 rows per
+// workgroup put 256 workgroups on an 8192 x 64 leaf (64 rows gave 128, half
+// the CUs; 16 measured 1.7% slower in the leaf pair, whose workgroups each
+// re-read two inverses and a block of op(A)).
+constexpr int kLeafRows = 32;
 template <typename T>
 __global__ __launch_bounds__(256) void trsm_apply_right(const double *inv, bool upper, int nb,
                                                         int64_t m, T *b, int64_t ldb) {
