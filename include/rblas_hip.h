@@ -83,7 +83,10 @@ const char *rblas_last_error_message(void);
  * Returns RBLAS_PANIC exactly where rblas's `gemm` would panic (the NN
  * asserts at level3.rs:164-166, the index checks of `materialize` at
  * level3.rs:179-222, the C slice in `apply_beta` at kernel/gemm.rs:327), and
- * also where rblas would write C out of bounds through a raw pointer. */
+ * also where rblas would write C out of bounds through a raw pointer.
+ * RBLAS_BAD_ARG when C's columns overlap (after the RowMajor swap: ldc < m
+ * with n > 1) and C is written, and for leading dimensions beyond the
+ * device's 31-bit tile offsets (ld * 128 * sizeof(T) >= 2^31). */
 int32_t rblas_sgemm(int32_t layout, int32_t transa, int32_t transb, size_t m,
                     size_t n, size_t k, float alpha, const float *a,
                     size_t a_len, size_t lda, const float *b, size_t b_len,

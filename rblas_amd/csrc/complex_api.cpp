@@ -52,6 +52,16 @@ Check cgemm_validate(int32_t ta, int32_t tb, size_t m, size_t n, size_t k, size_
   Check c = cgemm_panic_check(is_transposed(ta), is_transposed(tb), m, n, k, a_len, lda, b_len,
                               ldb, c_len, ldc);
   if (!c.ok()) return c;
+  // complex3.rs:61-73 writes every C entry in column order, so overlapping
+  // columns (ldc < m) see earlier columns' results; the device combine
+  // writes them at the same time.
+  if (m > 0 && n > 1 && ldc < m) {
+    Check bad;
+    bad.status = RBLAS_BAD_ARG;
+    bad.err = {RBLAS_BAD_ARG, 0, 0, 0};
+    bad.msg = "complex gemm: ldc < m (overlapping columns of C) is not supported on the device";
+    return bad;
+  }
   // The real gemms run with k' = 2k and leading dimensions m and 2k.
   if (m > INT_MAX || n > INT_MAX || 2 * k > INT_MAX ||
       ((m > 2 * k ? m : 2 * k) * 128 + 160) * sizeof(T) >= 0x7fffffe0ull) {

@@ -71,6 +71,8 @@ hipError_t enqueue(const ColMajorCall<T> &cc, const T *a, size_t lda, const T *b
 template <typename T>
 int32_t gemm_host(const ColMajorCall<T> &cc, int planes = 0) {
   if (cc.m == 0 || cc.n == 0) return RBLAS_OK;
+  // beta 1 with no product leaves C untouched (kernel/gemm.rs:202-205).
+  if ((cc.k == 0 || cc.alpha == T(0)) && cc.beta == T(1)) return RBLAS_OK;
   CtxLease lease;
   if (!lease.get()) return hip_fail(lease.status(), "stream create");
   Ctx &ctx = *lease.get();

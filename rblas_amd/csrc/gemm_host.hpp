@@ -51,14 +51,25 @@ inline Check normalize(int32_t layout, int32_t ta, int32_t tb, size_t m, size_t 
     bad.msg = "gemm: m, n and k must each be below 2^31 on the device path";
     return bad;
   }
-  // A 128-wide tile of either operand must span under 2 GiB (the tile
-  // loaders address it with 31-bit byte offsets).
-  const size_t ldmax = out.lda > out.ldb ? out.lda : out.ldb;
+  // rblas walks C's columns one after another, so columns that overlap
+  // (ldc < m) see each other's updates in that order; device tiles update
+  // them at the same time. Refused whenever C is written.
+  const bool touches_c = out.m && out.n && (out.beta != T(1) || (out.k && out.alpha != T(0)));
+  if (touches_c && out.n > 1 && out.ldc < out.m) {
+    bad.status = RBLAS_BAD_ARG;
+    bad.err = {RBLAS_BAD_ARG, 0, 0, 0};
+    bad.msg = "gemm: ldc < m (overlapping columns of C) is not supported on the device";
+    return bad;
+  }
+  // A 128-wide tile of either operand, and of C, must span under 2 GiB (the
+  // tile loaders and the epilogue address it with 31-bit byte offsets).
+  size_t ldmax = out.lda > out.ldb ? out.lda : out.ldb;
+  if (out.ldc > ldmax) ldmax = out.ldc;
   const bool runs_kernel = out.m && out.n && out.k && out.alpha != T(0);
   if (runs_kernel && (ldmax * 128 + 160) * sizeof(T) >= 0x7fffffe0ull) {
     bad.status = RBLAS_BAD_ARG;
     bad.err = {RBLAS_BAD_ARG, 0, 0, 0};
-    bad.msg = "gemm: lda/ldb above 2^31/(128*sizeof(T)) elements is not supported on the device";
+    bad.msg = "gemm: lda/ldb/ldc above 2^31/(128*sizeof(T)) elements is not supported on the device";
     return bad;
   }
   return Check{};

@@ -15,6 +15,12 @@
 //       of Anderson, "Algorithm 978: Safe scaling in the Level 1 BLAS", ACM
 //       TOMS 44(1), 2017): squares of small, medium and large magnitudes are
 //       summed in separately scaled accumulators, which add across threads.
+//       Non-finite input gives rblas's result: its running scale / ssq pair
+//       turns any NaN into NaN, and a second infinite entry makes inf / inf
+//       (NaN) at level1.rs:116, while a single infinity gives +Inf. So the
+//       partial kernels also count infinite entries, and the finish kernels
+//       return NaN when the count reaches two (a NaN entry already
+//       propagates through the sums).
 #include "kernels.hpp"
 #include "fma.hpp"
 
@@ -161,11 +167,22 @@ __global__ __launch_bounds__(kThreads) void axpy_kernel(int64_t n, T alpha, cons
 }
 
 // ------------------------------------------------------------------ nrm2
-// f32: sum of squares in f64.
+// 1 for an infinite entry, else 0 (NaN compares unequal to infinity).
+__device__ __forceinline__ double inf_count(double v) {
+  return __builtin_fabs(v) == __builtin_inf() ? 1.0 : 0.0;
+}
+
+// f32: sum of squares in f64. Partials: [2 b] = sum of squares, [2 b + 1] =
+// number of infinite entries.
+__device__ __forceinline__ void sq_add(double &acc, double &ninf, double v) {
+  acc = __builtin_fma(v, v, acc);
+  ninf += inf_count(v);
+}
+
 __global__ __launch_bounds__(kThreads) void snrm2_partial(int64_t n, const float *x, int64_t incx,
                                                           double *partial) {
   __shared__ double sh[kThreads / 64];
-  double acc = 0.0;
+  double acc = 0.0, ninf = 0.0;
   const int64_t tid = blockIdx.x * (int64_t)kThreads + threadIdx.x;
   const int64_t stride = (int64_t)gridDim.x * kThreads;
   if (incx == 1 && aligned16(x)) {
@@ -175,26 +192,31 @@ __global__ __launch_bounds__(kThreads) void snrm2_partial(int64_t n, const float
     for (int64_t i = tid; i < nv; i += stride) {
       const V a = xv[i];
 #pragma unroll
-      for (int e = 0; e < 4; ++e) acc = __builtin_fma((double)a[e], (double)a[e], acc);
+      for (int e = 0; e < 4; ++e) sq_add(acc, ninf, (double)a[e]);
     }
-    for (int64_t i = nv * 4 + tid; i < n; i += stride) acc = __builtin_fma((double)x[i], (double)x[i], acc);
+    for (int64_t i = nv * 4 + tid; i < n; i += stride) sq_add(acc, ninf, (double)x[i]);
   } else {
-    for (int64_t i = tid; i < n; i += stride) {
-      const double v = x[i * incx];
-      acc = __builtin_fma(v, v, acc);
-    }
+    for (int64_t i = tid; i < n; i += stride) sq_add(acc, ninf, (double)x[i * incx]);
   }
   const double s = block_sum(acc, sh);
-  if (threadIdx.x == 0) partial[blockIdx.x] = s;
+  const double c = block_sum(ninf, sh);
+  if (threadIdx.x == 0) {
+    partial[2 * blockIdx.x] = s;
+    partial[2 * blockIdx.x + 1] = c;
+  }
 }
 
 __global__ __launch_bounds__(kThreads) void snrm2_finish(int nparts, const double *partial,
                                                          float *out) {
   __shared__ double sh[kThreads / 64];
-  double acc = 0.0;
-  for (int i = threadIdx.x; i < nparts; i += kThreads) acc += partial[i];
+  double acc = 0.0, ninf = 0.0;
+  for (int i = threadIdx.x; i < nparts; i += kThreads) {
+    acc += partial[2 * i];
+    ninf += partial[2 * i + 1];
+  }
   const double s = block_sum(acc, sh);
-  if (threadIdx.x == 0) *out = (float)__builtin_sqrt(s);
+  const double c = block_sum(ninf, sh);
+  if (threadIdx.x == 0) *out = c >= 2.0 ? __builtin_nanf("") : (float)__builtin_sqrt(s);
 }
 
 // f64: Blue's scaling. Thresholds and scale factors for IEEE double:
@@ -204,8 +226,9 @@ struct Blue {
 };
 constexpr double kTsml = 0x1p-511, kTbig = 0x1p486, kSsml = 0x1p537, kSbig = 0x1p-538;
 
-__device__ __forceinline__ void blue_add(Blue &b, double v) {
+__device__ __forceinline__ void blue_add(Blue &b, double &ninf, double v) {
   const double a = __builtin_fabs(v);
+  ninf += inf_count(v);
   if (a > kTbig) {
     const double t = a * kSbig;
     b.big = __builtin_fma(t, t, b.big);
@@ -221,6 +244,7 @@ __global__ __launch_bounds__(kThreads) void dnrm2_partial(int64_t n, const doubl
                                                           int64_t incx, double *partial) {
   __shared__ double sh[kThreads / 64];
   Blue b{0.0, 0.0, 0.0};
+  double ninf = 0.0;
   const int64_t tid = blockIdx.x * (int64_t)kThreads + threadIdx.x;
   const int64_t stride = (int64_t)gridDim.x * kThreads;
   if (incx == 1 && aligned16(x)) {
@@ -229,34 +253,42 @@ __global__ __launch_bounds__(kThreads) void dnrm2_partial(int64_t n, const doubl
     const V *xv = (const V *)x;
     for (int64_t i = tid; i < nv; i += stride) {
       const V a = xv[i];
-      blue_add(b, a[0]);
-      blue_add(b, a[1]);
+      blue_add(b, ninf, a[0]);
+      blue_add(b, ninf, a[1]);
     }
-    for (int64_t i = nv * 2 + tid; i < n; i += stride) blue_add(b, x[i]);
+    for (int64_t i = nv * 2 + tid; i < n; i += stride) blue_add(b, ninf, x[i]);
   } else {
-    for (int64_t i = tid; i < n; i += stride) blue_add(b, x[i * incx]);
+    for (int64_t i = tid; i < n; i += stride) blue_add(b, ninf, x[i * incx]);
   }
   const double s0 = block_sum(b.sml, sh);
   const double s1 = block_sum(b.med, sh);
   const double s2 = block_sum(b.big, sh);
+  const double s3 = block_sum(ninf, sh);
   if (threadIdx.x == 0) {
-    partial[3 * blockIdx.x] = s0;
-    partial[3 * blockIdx.x + 1] = s1;
-    partial[3 * blockIdx.x + 2] = s2;
+    partial[4 * blockIdx.x] = s0;
+    partial[4 * blockIdx.x + 1] = s1;
+    partial[4 * blockIdx.x + 2] = s2;
+    partial[4 * blockIdx.x + 3] = s3;
   }
 }
 
 __global__ __launch_bounds__(kThreads) void dnrm2_finish(int nparts, const double *partial,
                                                          double *out) {
   __shared__ double sh[kThreads / 64];
-  double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+  double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
   for (int i = threadIdx.x; i < nparts; i += kThreads) {
-    a0 += partial[3 * i];
-    a1 += partial[3 * i + 1];
-    a2 += partial[3 * i + 2];
+    a0 += partial[4 * i];
+    a1 += partial[4 * i + 1];
+    a2 += partial[4 * i + 2];
+    a3 += partial[4 * i + 3];
   }
   double asml = block_sum(a0, sh), amed = block_sum(a1, sh), abig = block_sum(a2, sh);
+  const double ninf = block_sum(a3, sh);
   if (threadIdx.x != 0) return;
+  if (ninf >= 2.0) {  // rblas: inf / inf in the scaled sum of squares
+    *out = __builtin_nan("");
+    return;
+  }
   // Combine (Anderson 2017): the large accumulator dominates when present;
   // otherwise the small one is merged with the medium one without overflow.
   double scl, sumsq;
@@ -291,7 +323,7 @@ int blocks_for(int64_t n) {
 
 }  // namespace
 
-size_t level1_scratch_bytes() { return (size_t)kMaxBlocks * 3 * sizeof(double); }
+size_t level1_scratch_bytes() { return (size_t)kMaxBlocks * 4 * sizeof(double); }
 
 template <typename T>
 hipError_t launch_dot(int64_t n, const T *x, int64_t incx, const T *y, int64_t incy, T *out,

@@ -8,7 +8,7 @@ import re
 import numpy as np
 import pytest
 
-from rblas_amd import _ffi, level3
+from rblas_amd import _ffi, complex3, grouped, level3
 from rblas_amd.types import BlasError, BlasPanic
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -106,3 +106,59 @@ def test_gemm_zero_dims_need_no_device():
         c = np.full(6, 7.0)
         level3.gemm(layout, ta, tb, 0, 3, 2, 1.0, np.zeros(64), 3, np.zeros(64), 3, 0.0, c, 3)
         assert (c == 7).all()
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_gemm_overlapping_c_columns_refused(dtype):
+    """rblas updates C's columns one after another, so with ldc < m (column
+    major) or ldc < n (row major) overlapping columns see earlier columns'
+    results; a device cannot follow that order, so the call is refused with
+    RBLAS_BAD_ARG before any device work. A call that leaves C untouched
+    (beta 1 and alpha 0) stays a no-op."""
+    m, n, k = 6, 4, 3
+    a, b = np.ones(64, dtype), np.ones(64, dtype)
+    for layout, ldc in ((1, m - 1), (0, n - 1)):
+        lda = m if layout == 1 else k
+        ldb = k if layout == 1 else n
+        c = np.zeros(64, dtype)
+        for alpha, beta in ((1.0, 0.0), (0.0, 0.5), (1.0, 1.0)):
+            with pytest.raises(ValueError, match="overlapping columns"):
+                level3.gemm(layout, 0, 0, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc)
+        level3.gemm(layout, 0, 0, m, n, k, 0.0, a, lda, b, ldb, 1.0, c, ldc)
+        assert (c == 0).all()
+    # one column (n == 1) cannot overlap
+    with pytest.raises(Exception) as e:
+        level3.gemm(1, 0, 0, m, 1, k, 1.0, a, m, b, k, 0.0, np.zeros(64, dtype), 1)
+    assert "overlapping" not in str(e.value)
+
+
+def test_complex_gemm_overlapping_c_columns_refused():
+    m, n, k = 5, 3, 2
+    for dt in (np.complex64, np.complex128):
+        a, b, c = np.ones(32, dt), np.ones(32, dt), np.zeros(32, dt)
+        with pytest.raises(ValueError, match="overlapping columns"):
+            complex3.gemm(0, 0, m, n, k, 1.0, a, m, b, k, 0.0, c, m - 2)
+
+
+def test_gemm_large_ldc_refused_before_device():
+    """A 128-row tile of C must span under 2 GiB for the epilogue's 31-bit
+    offsets: ldc * 128 * 8 >= 2^31 is refused with RBLAS_BAD_ARG (not a HIP
+    error) before any device work."""
+    m, n, k = 4, 2, 3
+    ldc = (1 << 31) // (128 * 8) + 1
+    c = np.zeros(ldc + m)
+    with pytest.raises(ValueError, match="ldc"):
+        level3.gemm(1, 0, 0, m, n, k, 1.0, np.ones(m * k), m, np.ones(k * n), k, 0.0, c, ldc)
+
+
+def test_grouped_overlapping_c_columns_refused():
+    class _Host:
+        def __init__(self, size):
+            self.ptr, self.size, self.dtype = 0, size, np.dtype(np.float64)
+
+    ok = grouped.GemmProblem(1, 0, 0, 4, 4, 4, 1.0, _Host(16), 4, _Host(16), 4, 0.0, _Host(16), 4)
+    over = grouped.GemmProblem(1, 0, 0, 4, 4, 4, 1.0, _Host(16), 4, _Host(16), 4, 0.0,
+                               _Host(16), 3)
+    with pytest.raises(ValueError) as e:
+        grouped.GroupedGemm([ok, over], np.float64)
+    assert "problem 1" in str(e.value) and "overlapping" in str(e.value)

@@ -119,6 +119,63 @@ def test_nrm2_extremes(gpu):
     assert abs(level1.nrm2(4, f, 1) - 2e38) / 2e38 < 1e-6
 
 
+NONFINITE_CASES = {
+    "inf": ([np.inf], "inf"),
+    "inf_finite": ([3.0, np.inf, 4.0], "inf"),
+    "neg_inf": ([1.0, -np.inf], "inf"),
+    "two_inf": ([np.inf, np.inf], "nan"),
+    "inf_1_neginf": ([np.inf, 1.0, -np.inf], "nan"),
+    "three_inf": ([-np.inf, 2.0, np.inf, 0.0, np.inf], "nan"),
+    "nan": ([1.0, np.nan, 2.0], "nan"),
+    "inf_nan": ([np.inf, np.nan], "nan"),
+    "nan_inf": ([0.0, np.nan, 5.0, -np.inf], "nan"),
+    "zeros": ([0.0, 0.0], "zero"),
+}
+
+
+def nonfinite_vector(vals, dtype, n, spread):
+    """vals placed at spread positions of an n-vector of N(0,1) entries
+    (far apart, so they land in different workgroups of the device
+    reduction), or the bare values when n == len(vals)."""
+    if n == len(vals):
+        return np.array(vals, dtype)
+    x = np.random.default_rng(n).standard_normal(n).astype(dtype)
+    pos = np.linspace(0, n - 1, len(vals)).astype(np.int64) if spread else np.arange(len(vals))
+    x[pos] = vals
+    return x
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("case", sorted(NONFINITE_CASES))
+def test_nrm2_nonfinite_matches_oracle(gpu, oracle, dtype, case):
+    """level1.rs:96-123 on non-finite input: any NaN gives NaN, a second
+    infinity gives inf / inf = NaN at level1.rs:116, one infinity gives +Inf.
+    The GPU must return the oracle's class of result on short vectors,
+    strided vectors and long vectors whose special entries sit in different
+    workgroups."""
+    vals, want = NONFINITE_CASES[case]
+    for n, incx, spread in ((len(vals), 1, False), (4099, 1, True), (2_000_003, 1, True),
+                            (1001, 3, True)):
+        base = nonfinite_vector(vals, dtype, n, spread)
+        if incx > 1:
+            x = np.full((n - 1) * incx + 1, np.nan, dtype)  # gaps hold NaN, never read
+            x[::incx] = base
+        else:
+            x = base
+        got = level1.nrm2(n, x, incx)
+        ref = oracle.nrm2(n, x, incx)
+        if want == "nan":
+            assert math.isnan(got) and math.isnan(ref), (n, incx, got, ref)
+        elif want == "inf":
+            assert got == math.inf and ref == math.inf, (n, incx, got, ref)
+        elif n == len(vals):
+            assert got == 0 and ref == 0
+        else:
+            u = 2.0 ** -24 if dtype == np.float32 else 2.0 ** -53
+            assert abs(got - ref) <= 4 * (n + 2) * u * ref, (n, incx, got, ref)
+
+
 @pytest.mark.gpu
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
 def test_level1_device_entries(gpu, dtype):

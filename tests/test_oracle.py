@@ -284,6 +284,26 @@ def test_level1_kats(oracle):
 
 
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_nrm2_nonfinite_oracle(oracle, dtype):
+    """level1.rs:103-122 traced by hand under IEEE rules: the first infinity
+    takes the new-max branch (r = scale / inf = 0, ssq = 1, scale = inf); a
+    later finite entry adds (a / inf)^2 = 0; a second infinity takes the
+    other branch (inf < inf is false) with r = inf / inf = NaN, so ssq and
+    the result are NaN. A NaN entry fails scale < NaN and makes r = NaN."""
+    def nrm(v):
+        return oracle.nrm2(len(v), np.array(v, dtype), 1)
+    assert nrm([np.inf]) == math.inf
+    assert nrm([2.0, -np.inf, 7.0]) == math.inf
+    assert math.isnan(nrm([np.inf, np.inf]))
+    assert math.isnan(nrm([np.inf, 1.0, -np.inf]))
+    assert math.isnan(nrm([np.nan]))
+    assert math.isnan(nrm([3.0, np.nan, 4.0]))
+    assert math.isnan(nrm([np.inf, np.nan]))
+    assert math.isnan(nrm([np.nan, np.inf]))
+    assert nrm([0.0, 0.0]) == 0.0
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
 def test_level1_vs_numpy(oracle, dtype):
     rng = np.random.default_rng(2)
     for n in (0, 1, 7, 15, 16, 17, 1000, 4099):
