@@ -13,6 +13,9 @@ consecutive offsets.
 """
 from __future__ import annotations
 
+import concurrent.futures as cf
+import os
+
 import numpy as np
 
 GOLDEN = np.uint64(0x9E3779B97F4A7C15)
@@ -30,7 +33,34 @@ def splitmix64(seed: int, offset: int, count: int) -> np.ndarray:
         return z ^ (z >> np.uint64(31))
 
 
-def uniform(seed: int, offset: int, count: int, dtype) -> np.ndarray:
+_CHUNK = 1 << 22  # elements per worker task (even, so normal pairs stay aligned)
+
+
+def _parallel(fn, seed: int, offset: int, count: int, dtype) -> np.ndarray | None:
+    """fn's draw computed in chunks on host threads (numpy's loops release
+    the GIL). The stream is counter-based, so chunk j is the draw at offset
+    + j * _CHUNK and the result equals the one-piece draw. None for small
+    draws."""
+    if count <= 2 * _CHUNK:
+        return None
+    out = np.empty(count, dtype)
+    starts = range(0, count, _CHUNK)
+
+    def one(c0):
+        c1 = min(count, c0 + _CHUNK)
+        out[c0:c1] = fn(seed, offset + c0, c1 - c0, dtype, _whole=True)
+
+    workers = min(16, os.cpu_count() or 1)
+    with cf.ThreadPoolExecutor(max_workers=workers) as ex:
+        list(ex.map(one, starts))
+    return out
+
+
+def uniform(seed: int, offset: int, count: int, dtype, _whole: bool = False) -> np.ndarray:
+    if not _whole:
+        big = _parallel(uniform, seed, offset, count, dtype)
+        if big is not None:
+            return big
     z = splitmix64(seed, offset, count)
     if np.dtype(dtype) == np.float32:
         u = (z >> np.uint64(40)).astype(np.float32) * np.float32(2.0 ** -24)
@@ -39,7 +69,11 @@ def uniform(seed: int, offset: int, count: int, dtype) -> np.ndarray:
     return (2.0 * u - 1.0).astype(dtype)
 
 
-def normal(seed: int, offset: int, count: int, dtype) -> np.ndarray:
+def normal(seed: int, offset: int, count: int, dtype, _whole: bool = False) -> np.ndarray:
+    if not _whole:
+        big = _parallel(normal, seed, offset, count, dtype)
+        if big is not None:
+            return big
     pairs = (count + 1) // 2
     z = splitmix64(seed, offset, 2 * pairs)
     u1 = ((z[0::2] >> np.uint64(11)).astype(np.float64) + 1.0) * 2.0 ** -53

@@ -276,3 +276,32 @@ def test_gemm_split_k_exact_and_deterministic(gpu, dtype, m, n, k):
     assert np.array_equal(c1, c2)
     want, mag = gemm_expect(COL, 0, 1, m, n, k, 1.5, a, lda, b, ldb, -0.25, c, ldc)
     assert_gemm_close(view(c1, COL, m, n, ldc), want, mag, k, dtype, what="split-k")
+
+
+def test_try_gemm_valid_call_succeeds_reference(gpu):
+    # level3.rs:574-591: the valid 4 x 3 x 5 call of ones returns Ok; every
+    # entry of C is then k = 5.
+    m, n, k = 4, 3, 5
+    a, b, c = np.ones(m * k), np.ones(k * n), np.zeros(m * n)
+    level3.try_gemm(COL, 0, 0, m, n, k, 1.0, a, m, b, k, 0.0, c, m)
+    assert (c == k).all()
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_try_gemm_ok_equals_gemm_bitwise(gpu, dtype):
+    """Where validation passes, try_gemm runs the same gemm: C is equal bit
+    for bit to level3.gemm's on the same inputs, over layouts, transposes,
+    padded leading dimensions, exact-length slices and alpha / beta cases."""
+    rng = np.random.default_rng(21)
+    checked = 0
+    for (layout, ta, tb), (m, n, k), pad, (alpha, beta) in itertools.product(
+            LAYOUTS[::2], [(8, 8, 8), (37, 5, 130), (129, 66, 1)], (0, 3),
+            [(1.5, -0.25), (1.0, 0.0), (0.0, 2.0)]):
+        a, lda, b, ldb, c, ldc = make_problem(layout, ta, tb, m, n, k, pad, dtype,
+                                              int(rng.integers(1 << 30)))
+        c_try, c_gemm = c.copy(), c.copy()
+        level3.try_gemm(layout, ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c_try, ldc)
+        level3.gemm(layout, ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c_gemm, ldc)
+        assert np.array_equal(c_try, c_gemm, equal_nan=True)
+        checked += 1
+    assert checked == 9 * 3 * 2 * 3

@@ -394,3 +394,74 @@ def test_trsm_exact_integers_large(gpu, q, side):
     b = B.T.reshape(-1).copy()
     level3.trsm(side, 1, 0, 0, m, n, 1.0, a, q, b, m)
     assert np.array_equal(b.reshape(n, m).T, X)
+
+
+def _ill_conditioned_lower(kind, q, rng):
+    """Lower triangular q x q (float64): 'unit' has a unit diagonal and N(0,1)
+    off-diagonals (condition number growing like 2^q); 'spread' has
+    diagonals log-uniform over [1e-3, 1e3] and N(0,1)/sqrt(q) below."""
+    if kind == "unit":
+        return np.tril(rng.standard_normal((q, q)), -1) + np.eye(q)
+    L = np.tril(rng.standard_normal((q, q)), -1) / np.sqrt(q)
+    np.fill_diagonal(L, np.exp(rng.uniform(np.log(1e-3), np.log(1e3), q)))
+    return L
+
+
+def _solve_longdouble(T, C, lower):
+    """T Y = C by substitution in 80-bit extended precision (x87 long
+    double): the forward-error reference."""
+    q = T.shape[0]
+    Tl = T.astype(np.longdouble)
+    Y = C.astype(np.longdouble).copy()
+    order = range(q) if lower else range(q - 1, -1, -1)
+    for i in order:
+        rest = slice(0, i) if lower else slice(i + 1, q)
+        Y[i] = (Y[i] - Tl[i, rest] @ Y[rest]) / Tl[i, i]
+    return Y
+
+
+ILL_CASES = [(np.float64, q, kind) for q in (64, 200, 1000) for kind in ("unit", "spread")] + \
+    [(np.float32, 64, "unit"), (np.float32, 200, "unit"), (np.float32, 64, "spread")]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("dtype,q,kind", ILL_CASES)
+@pytest.mark.parametrize("side", [0, 1])
+def test_trsm_ill_conditioned_vs_substitution(gpu, oracle, dtype, q, kind, side):
+    """Ill-conditioned triangular factors, where an explicit inverse of a
+    diagonal block is less stable than rblas's substitution
+    (level3.rs:440-520, unfused s - a x, then / a_ii). The GPU must stay
+    within the residual bound 16 q u (|op(A)||X| + |B|) and within 16x the
+    oracle's forward error (floor q u) against an extended-precision solve.
+    Left: L X = B (forward substitution); Right: X L = B, i.e. L^T X^T =
+    B^T (backward). float32 runs the sizes whose solutions stay finite in
+    float32 (the 'unit' solution reaches ~1e114 at q = 1000, the 'spread'
+    one ~1e58 at q = 200)."""
+    rng = np.random.default_rng(q + (kind == "unit") * 7 + side * 13)
+    L = _ill_conditioned_lower(kind, q, rng).astype(dtype)
+    nrhs = 8
+    m, n = (q, nrhs) if side == 0 else (nrhs, q)
+    B = rng.standard_normal((m, n)).astype(dtype)
+    a = L.T.reshape(-1).copy()  # column-major, lda = q, uplo Lower
+    b = B.T.reshape(-1).copy()  # column-major, ldb = m
+    b_ref = b.copy()
+    level3.trsm(side, 1, 0, 0, m, n, 1.0, a, q, b, m)
+    oracle.trsm(side, 1, 0, 0, m, n, 1.0, a, q, b_ref, m)
+    L64 = L.astype(np.float64)
+    B64 = B.astype(np.float64)
+    if side == 0:
+        X_true = _solve_longdouble(L64, B64, lower=True)
+    else:
+        X_true = _solve_longdouble(L64.T, B64.T, lower=False).T
+    scale = float(np.abs(X_true).max())
+    u = 2.0 ** -24 if dtype == np.float32 else 2.0 ** -53
+    out = {}
+    for what, got in (("gpu", b), ("oracle", b_ref)):
+        X = got.reshape(n, m).T.astype(np.float64)
+        assert np.isfinite(X).all(), what
+        out[what + "_resid"] = _residual_ok(side, L64, X, 1.0, B64, q, dtype)
+        out[what + "_ferr"] = float(np.abs(X - X_true.astype(np.float64)).max() / scale)
+    print({k: f"{v:.3g}" for k, v in out.items()}, f"max|X| {scale:.3g}")
+    assert out["oracle_resid"] <= 1.0
+    assert out["gpu_resid"] <= 1.0
+    assert out["gpu_ferr"] <= 16 * max(out["oracle_ferr"], q * u)

@@ -59,7 +59,8 @@ def assert_gemm_close(got, want, mag, k, dtype, c_const=2.0, what=""):
     forward bound for a length-k dot product plus the alpha/beta update)."""
     u = unit_roundoff(dtype)
     tol = c_const * (k + 2) * u * mag + np.finfo(np.float64).tiny
-    err = np.abs(got.astype(np.float64) - want)
+    wide = np.complex128 if np.iscomplexobj(got) or np.iscomplexobj(want) else np.float64
+    err = np.abs(got.astype(wide) - want)
     bad = err > tol
     if bad.any():
         idx = np.argwhere(bad)[0]
