@@ -127,6 +127,24 @@ int32_t rblas_dgemm_dev(int32_t layout, int32_t transa, int32_t transb,
                         double beta, double *c, size_t c_len, size_t ldc,
                         void *stream);
 
+/* Device-pointer DGEMM that allocates nothing. No counterpart in rblas: it
+ * is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc). Same
+ * arguments, validation, launch shape and result bits as rblas_dgemm_dev;
+ * the k-split partial tiles and counters live in a fixed 64 MiB device array
+ * that each GPU holds until the process ends, instead of the stream-ordered
+ * pool. Its size covers every k-split shape the tile kernel picks for f64;
+ * a call whose workspace would exceed it returns RBLAS_BAD_ARG. All calls on
+ * one GPU share the array: calls that overlap on the device (on different
+ * streams, or from different threads) race on it and must be serialized by
+ * the caller, e.g. by issuing them on one stream. */
+int32_t rblas_dgemm_dev_noalloc(int32_t layout, int32_t transa,
+                                int32_t transb, size_t m, size_t n, size_t k,
+                                double alpha, const double *a, size_t a_len,
+                                size_t lda, const double *b, size_t b_len,
+                                size_t ldb, double beta, double *c,
+                                size_t c_len, size_t ldc, void *stream);
+
 /* SGEMM with a selectable arithmetic (no rblas counterpart: rblas's SGEMM
  * is plain f32). `compute` is one of:
  *   RBLAS_COMPUTE_F32     plain f32 MFMA (what rblas_sgemm does)

@@ -108,6 +108,7 @@ SIGNATURES: dict[str, tuple[list, object]] = {
     "rblas_sgemm_ex": (_gemm_args(f32) + [i32], i32),
     "rblas_sgemm_ex_dev": (_gemm_args(f32) + [i32, vp], i32),
     "rblas_dgemm_dev": (_gemm_args(f64) + [vp], i32),
+    "rblas_dgemm_dev_noalloc": (_gemm_args(f64) + [vp], i32),
     "rblas_cgemm": ([i32, i32, sz, sz, sz, Complex32, vp, sz, sz, vp, sz, sz, Complex32, vp, sz,
                      sz], i32),
     "rblas_zgemm": ([i32, i32, sz, sz, sz, Complex64, vp, sz, sz, vp, sz, sz, Complex64, vp, sz,

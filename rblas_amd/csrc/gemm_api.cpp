@@ -1,6 +1,8 @@
 // C-ABI gemm entry points: level3::gemm / level3::try_gemm (level3.rs:32-103)
 // for host slices, plus the HBM-resident `_dev` form.
 #include <climits>
+#include <mutex>
+#include <string>
 
 #include "gemm_host.hpp"
 #include "kernels.hpp"
@@ -283,12 +285,84 @@ int32_t symm_dev(int32_t side, int32_t uplo, size_t m, size_t n, T alpha, const 
   return RBLAS_OK;
 }
 
+// ------------------------------------------------- no-allocation DGEMM
+// The k-split partial tiles and per-tile counters of rblas_dgemm_dev_noalloc
+// live in this fixed device array instead of the stream-ordered pool. Its
+// size covers every split configuration gemm_config gives f64: a split runs
+// only with fewer than 256 tiles of 128^2 (or 128 of 64^2) and at most
+// ceil(256 / tiles) splits, so tiles x splits <= 510 at 128^2 (255 x 2) and
+// <= 383 at 64^2: 510 x 128 x 128 x 8 bytes = 63.75 MiB of partials, plus
+// at most 255 counters. Every GPU the library runs on holds one copy until
+// the process ends; calls on one GPU share it.
+constexpr size_t kNoAllocScratchBytes = size_t(64) << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kMaxDevices] = {};
+
+// The device address of `scratch` on the current device, looked up once per
+// device (the array's name is only a host-side stand-in).
+hipError_t noalloc_scratch(void **out) {
+  *out = nullptr;
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> g(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void *ptr = nullptr;
+    e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    if (!ptr) return hipErrorInvalidSymbol;
+    g_scratch_ptr[dev] = ptr;
+  }
+  *out = g_scratch_ptr[dev];
+  return hipSuccess;
+}
+
+int32_t dgemm_dev_noalloc(int32_t layout, int32_t ta, int32_t tb, size_t m, size_t n, size_t k,
+                          double alpha, const double *a, size_t a_len, size_t lda, const double *b,
+                          size_t b_len, size_t ldb, double beta, double *c, size_t c_len,
+                          size_t ldc, void *stream) {
+  ColMajorCall<double> cc;
+  Check chk = normalize(layout, ta, tb, m, n, k, alpha, a, a_len, lda, b, b_len, ldb, beta, c,
+                        c_len, ldc, cc);
+  if (!chk.ok()) return fail(chk, nullptr);
+  const size_t wsb = workspace_bytes(cc, 0);
+  if (wsb > kNoAllocScratchBytes) {
+    Check bad;
+    bad.status = RBLAS_BAD_ARG;
+    bad.err = {RBLAS_BAD_ARG, 0, 0, 0};
+    bad.msg = "dgemm_dev_noalloc: the k-split workspace (" + std::to_string(wsb) +
+              " bytes) exceeds the fixed device scratch (" +
+              std::to_string(kNoAllocScratchBytes) + " bytes)";
+    return fail(bad, nullptr);
+  }
+  void *ws = nullptr;
+  if (wsb) RBLAS_HIP_TRY(noalloc_scratch(&ws), "dgemm_dev_noalloc scratch address");
+  // The counters are cleared by a fill kernel inside the launch, and the
+  // partials are written before they are read, so a previous call's
+  // contents never reach this one.
+  RBLAS_HIP_TRY(enqueue(cc, cc.a, cc.lda, cc.b, cc.ldb, cc.c, cc.ldc, 0, ws, (hipStream_t)stream),
+                "dgemm_dev_noalloc launch");
+  return RBLAS_OK;
+}
+
 }  // namespace
 }  // namespace rblas
 
 using namespace rblas;
 
 extern "C" {
+
+int32_t rblas_dgemm_dev_noalloc(int32_t layout, int32_t transa, int32_t transb, size_t m,
+                                size_t n, size_t k, double alpha, const double *a, size_t a_len,
+                                size_t lda, const double *b, size_t b_len, size_t ldb,
+                                double beta, double *c, size_t c_len, size_t ldc, void *stream) {
+  return dgemm_dev_noalloc(layout, transa, transb, m, n, k, alpha, a, a_len, lda, b, b_len, ldb,
+                           beta, c, c_len, ldc, stream);
+}
 
 int32_t rblas_sgemm(int32_t layout, int32_t transa, int32_t transb, size_t m, size_t n, size_t k,
                     float alpha, const float *a, size_t a_len, size_t lda, const float *b,

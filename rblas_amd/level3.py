@@ -98,6 +98,19 @@ def gemm_dev(layout, transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ld
     raise_status(st)
 
 
+def dgemm_dev_noalloc(layout, transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc,
+                      stream=None) -> None:
+    """f64 gemm_dev that allocates no device memory (its k-split workspace
+    is a fixed device array shared by every call on the GPU: calls must not
+    overlap on the device)."""
+    if _dev_prefix(a, b, c) != "d":
+        raise TypeError("dgemm_dev_noalloc takes float64 operands")
+    st = _ffi.lib().rblas_dgemm_dev_noalloc(
+        int(layout), int(transa), int(transb), m, n, k, float(alpha), a.ptr, a.size, lda, b.ptr,
+        b.size, ldb, float(beta), c.ptr, c.size, ldc, stream.handle if stream is not None else None)
+    raise_status(st)
+
+
 def sgemm_ex(layout, transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc,
              compute) -> None:
     """f32 gemm with a chosen arithmetic (types.Compute); host slices."""
