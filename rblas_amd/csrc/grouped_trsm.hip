@@ -9,11 +9,12 @@
 // B for Right). The unit keeps its W right-hand sides in LDS as f64
 // (q x W, q the order of A, q * W <= 16384), then walks the diagonal blocks
 // in substitution order:
-//   X_b = inv_b * B_b   (f64 MFMA over the strictly off-diagonal 16 x 16
-//                        sub-blocks of inv_b, scalar f64 over the triangle of
-//                        each diagonal sub-block, so no structural zero ever
-//                        multiplies a B entry: a non-finite entry reaches
-//                        exactly what rblas's substitution reaches)
+//   X_b = inv_b * B_b   (f64 MFMA over the 16 x 16 sub-blocks of inv_b; a
+//                        diagonal sub-block whose B rows hold a non-finite
+//                        entry is summed over its triangle only, scalar f64,
+//                        so no structural zero ever multiplies it: a
+//                        non-finite entry reaches exactly what rblas's
+//                        substitution reaches)
 //   B_rest -= op(A)[rest, b] * X_b   (f64 MFMA, op(A) streamed from HBM/L2)
 // and writes the right-hand sides back. All arithmetic is f64 (f32 operands
 // are widened on load and rounded once on store).
@@ -56,17 +57,80 @@ __device__ __forceinline__ double lane_value(double v, int l) {
   return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
 }
 
+// Lane S's value of each row of 16 lanes, in every lane of the row (DPP
+// row broadcast); S must be a constant once the caller's loop is unrolled.
+template <int S>
+__device__ __forceinline__ int row_bcast_i(int v) {
+  return __builtin_amdgcn_update_dpp(0, v, 0x150 + S, 0xf, 0xf, false);
+}
+template <int S>
+__device__ __forceinline__ double row_bcast_d(double v) {
+  const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+  const unsigned lo = (unsigned)row_bcast_i<S>((int)(unsigned)u);
+  const unsigned hi = (unsigned)row_bcast_i<S>((int)(unsigned)(u >> 32));
+  return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+__device__ __forceinline__ double row_bcast(double v, int s) {
+  switch (s) {
+    case 0: return row_bcast_d<0>(v);
+    case 1: return row_bcast_d<1>(v);
+    case 2: return row_bcast_d<2>(v);
+    case 3: return row_bcast_d<3>(v);
+    case 4: return row_bcast_d<4>(v);
+    case 5: return row_bcast_d<5>(v);
+    case 6: return row_bcast_d<6>(v);
+    case 7: return row_bcast_d<7>(v);
+    case 8: return row_bcast_d<8>(v);
+    case 9: return row_bcast_d<9>(v);
+    case 10: return row_bcast_d<10>(v);
+    case 11: return row_bcast_d<11>(v);
+    case 12: return row_bcast_d<12>(v);
+    case 13: return row_bcast_d<13>(v);
+    case 14: return row_bcast_d<14>(v);
+    default: return row_bcast_d<15>(v);
+  }
+}
+
+typedef double LdsMat[kNB][kNB + 1];  // M[col][row], 64 x 64
+
+// acc += X[rx.., cx..] * Y[ry.., cy..] for 16 x 16 blocks of LDS matrices
+// (f64 MFMA, k in steps of 4). Output layout: acc[r] holds row
+// (lane >> 4) + 4 r, column lane & 15 -- also the B-operand layout of the
+// next product's k step r.
+__device__ __forceinline__ f64x4 mm16(const LdsMat &X, int rx, int cx, const LdsMat &Y, int ry,
+                                      int cy, f64x4 acc, int lane) {
+#pragma unroll
+  for (int t4 = 0; t4 < 4; ++t4) {
+    const int m = 4 * t4 + (lane >> 4);
+    acc = mma64(X[cx + m][rx + (lane & 15)], Y[cy + (lane & 15)][ry + m], acc);
+  }
+  return acc;
+}
+
 // Inverse of diagonal block `blk` of op(A) (as the problem is solved: Left
 // form), f64, column-major 64 x 64 at inv + d.inv0 + blk * 4096, zero outside
 // the block's nb x nb corner. An upper block is inverted as the lower one
-// P U P (P reverses the nb indices). Column-oriented substitution against
-// the identity, in registers: lane i holds row i, wave w columns 4w..4w+3;
-// step k scales row k by 1 / L(k, k) and subtracts L(i, k) times it from
-// every later row i. Column j only takes part from step j on (earlier steps
-// would only subtract multiples of its zeros).
+// P U P (P reverses the nb indices).
+//
+// When every entry of the block's triangle is finite, blockwise with MFMA:
+// the four 16 x 16 diagonal inverses D_b = L_bb^-1 by substitution (wave b;
+// lane (i, g) holds row i of columns g, g + 4, g + 8, g + 12; step k scales
+// row k by 1 / L(k, k), broadcasts it across the 16-lane row and subtracts
+// L(i, k) times it from every later row i), then the off-diagonal blocks by
+// distance d = i - j = 1, 2, 3 (i > j):
+//   inv_ij = -D_i (sum_{k=j}^{i-1} L_ik inv_kj),
+// each a handful of 16 x 16 f64 MFMA products, one wave per block.
+//
+// Otherwise (an Inf or NaN in the triangle) the whole block by
+// column-oriented substitution against the identity, in registers: lane i
+// holds row i, wave w columns 4w..4w+3; step k scales row k by 1 / L(k, k)
+// and subtracts L(i, k) times it from every later row i, so each entry gets
+// exactly the operations of a row-by-row substitution of its column and a
+// non-finite entry reaches no more of the inverse than it does there.
 template <typename T>
 __device__ __forceinline__ void invert_block(const TrsmDesc<T> &d, int blk, double *inv) {
-  __shared__ double Ls[kNB][kNB + 1];  // Ls[k][i] = L(i, k), lower form
+  __shared__ LdsMat Ls;  // Ls[k][i] = L(i, k), lower form
+  __shared__ LdsMat Iv;  // Iv[j][i] = inverse(i, j), lower form (MFMA path)
   const int64_t r0 = (int64_t)blk * kNB;
   const int nb = (int)min((int64_t)kNB, (int64_t)d.q - r0);
   const int t = threadIdx.x;
@@ -82,16 +146,70 @@ __device__ __forceinline__ void invert_block(const TrsmDesc<T> &d, int blk, doub
     const T *p = d.trans ? d.a + c + r * d.lda : d.a + r + c * d.lda;
     raw[u] = *((in || diag) ? p : d.a);
   }
+  bool fin = true;
 #pragma unroll
   for (int u = 0; u < kFill; ++u) {
     const int idx = t + u * kInvThreads;
     const int i = idx % kNB, k = idx / kNB;
     const bool in = i < nb && k < nb && i > k;
     const bool diag = i == k && i < nb && !d.unit;
-    Ls[k][i] = (in || diag) ? (double)raw[u] : (i == k ? 1.0 : 0.0);  // padding: identity
+    const double v = (in || diag) ? (double)raw[u] : (i == k ? 1.0 : 0.0);  // padding: identity
+    fin = fin && __builtin_isfinite(v);
+    Ls[k][i] = v;
   }
-  __syncthreads();
+  const bool all_fin = __syncthreads_and(fin);  // also publishes Ls
   const int lane = t & 63, w = uni(t >> 6);
+  double *out = inv + d.inv0 + (int64_t)blk * kNB * kNB;
+  if (all_fin) {
+    if (w < 4) {  // D_w by substitution
+      const int i = lane & 15, g = lane >> 4, o = 16 * w;
+      const double rl = 1.0 / Ls[o + i][o + i];
+      double x[4];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) x[r] = i == g + 4 * r ? 1.0 : 0.0;
+#pragma unroll
+      for (int k = 0; k < 16; ++k) {
+        const double lik = Ls[o + k][o + i];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const double y = x[r] * rl;
+          const double xk = row_bcast(y, k);
+          const double upd = __builtin_fma(-lik, xk, x[r]);
+          x[r] = i == k ? y : (i > k ? upd : x[r]);
+        }
+      }
+#pragma unroll
+      for (int r = 0; r < 4; ++r) Iv[o + g + 4 * r][o + i] = x[r];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int dist = 1; dist < 4; ++dist) {
+      if (w < 4 - dist) {  // block (i, j) = (w + dist, w)
+        const int bi = w + dist, bj = w;
+        f64x4 tacc = {0, 0, 0, 0};
+        for (int bk = bj; bk < bi; ++bk) tacc = mm16(Ls, 16 * bi, 16 * bk, Iv, 16 * bk, 16 * bj, tacc, lane);
+        f64x4 pacc = {0, 0, 0, 0};
+#pragma unroll
+        for (int t4 = 0; t4 < 4; ++t4) {
+          const int m = 4 * t4 + (lane >> 4);
+          pacc = mma64(Iv[16 * bi + m][16 * bi + (lane & 15)], tacc[t4], pacc);
+        }
+#pragma unroll
+        for (int r = 0; r < 4; ++r) Iv[16 * bj + (lane & 15)][16 * bi + (lane >> 4) + 4 * r] = -pacc[r];
+      }
+      __syncthreads();
+    }
+#pragma unroll
+    for (int u = 0; u < kFill; ++u) {
+      const int idx = t + u * kInvThreads;
+      const int li = idx % kNB, lj = idx / kNB;
+      const bool in = li < nb && lj < nb;
+      const int i = (d.upper && in) ? nb - 1 - li : li;
+      const int j = (d.upper && in) ? nb - 1 - lj : lj;
+      out[i + j * kNB] = (in && li / 16 >= lj / 16) ? Iv[lj][li] : 0.0;
+    }
+    return;
+  }
   const int j0 = 4 * w;
   const double rl = 1.0 / Ls[lane][lane];
   double x[4];
@@ -111,7 +229,6 @@ __device__ __forceinline__ void invert_block(const TrsmDesc<T> &d, int blk, doub
       }
     }
   }
-  double *out = inv + d.inv0 + (int64_t)blk * kNB * kNB;
 #pragma unroll
   for (int c = 0; c < 4; ++c) {
     const int jl = j0 + c;
@@ -298,22 +415,41 @@ __device__ void solve_unit(const TrsmDesc<T> &d, int c0, const double *inv, doub
           for (int t4 = 0; t4 < 4; ++t4) acc = mma64(pa[t4], qa[t4], acc);
         }
       }
-      // Diagonal sub-block: only the triangle, scalar f64 (predicated, so
-      // every lane runs the same 16 steps).
+      // Diagonal sub-block. Its inverse holds exact zeros outside the
+      // triangle, so when this sub-block's B rows are all finite the full
+      // 16 x 16 product (4 MFMAs) adds only exact zeros beyond the
+      // triangle's terms. A non-finite entry must not reach entries that do
+      // not depend on it (0 * Inf is NaN), so then the triangle is summed
+      // alone, scalar f64 (predicated: every lane runs the same 16 steps),
+      // and each X entry gets exactly the terms rblas's substitution gives it.
       const double *sd = Is + sub_slot(upper, iw, iw);
+      double pd[4], qd[4];
+      bool fin = true;
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int il = kq + 4 * r;  // row within the sub-block (MFMA output layout)
-        double sum = acc[r];
+      for (int t4 = 0; t4 < 4; ++t4) {
+        const int j = 4 * t4 + kq;
+        pd[t4] = sd[(lane & 15) + 16 * j];
+        qd[t4] = (col < W) ? Bs[(r0 + 16 * iw + j) * W + col] : 0.0;
+        fin = fin && __builtin_isfinite(qd[t4]);
+      }
+      if (__all(fin)) {
 #pragma unroll
-        for (int jl = 0; jl < 16; ++jl) {
-          const int j = 16 * iw + jl;
-          const bool on = (upper ? jl >= il : jl <= il) && j < nb && col < W;
-          const double w = on ? sd[il + 16 * jl] : 0.0;
-          const double x = on ? Bs[(r0 + j) * W + col] : 0.0;
-          sum = on ? __builtin_fma(w, x, sum) : sum;
+        for (int t4 = 0; t4 < 4; ++t4) acc = mma64(pd[t4], qd[t4], acc);
+      } else {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int il = kq + 4 * r;  // row within the sub-block (MFMA output layout)
+          double sum = acc[r];
+#pragma unroll
+          for (int jl = 0; jl < 16; ++jl) {
+            const int j = 16 * iw + jl;
+            const bool on = (upper ? jl >= il : jl <= il) && j < nb && col < W;
+            const double w = on ? sd[il + 16 * jl] : 0.0;
+            const double x = on ? Bs[(r0 + j) * W + col] : 0.0;
+            sum = on ? __builtin_fma(w, x, sum) : sum;
+          }
+          acc[r] = sum;
         }
-        acc[r] = sum;
       }
     }
     __syncthreads();  // every wave has read B_b and inv_b
