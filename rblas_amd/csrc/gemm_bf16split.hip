@@ -70,12 +70,17 @@ __device__ __forceinline__ char *lds_b() {
   else return g_sb2;
 }
 
-// LDS byte offset of plane s, tile row r (0..255), 16-byte k-chunk c (0..1):
-// rows are 32 B, so 64 lanes reading rows r..r+31 at chunks 0 and 1 read one
-// contiguous, bank-conflict-free kilobyte, in exactly the order the DMA
-// writes it.
+// LDS byte offset of plane s, tile row r (0..255), 16-byte k-chunk c (0..1).
+// Rows are 32 B; the two chunks of a row swap places when bit 2 xor bit 3 of
+// the row is set. A fragment read has lanes 0..31 on rows r..r+31 at one
+// chunk: unswizzled, rows 8 apart (64 banks of 4 B) or 4 apart (32 banks)
+// hit the same banks, a 2-way conflict on every read; with the swap, any 16
+// consecutive rows (8 for 32 banks) cover distinct banks. The DMA writes
+// lane l's 16 bytes at position l of a 1 KB block, so each lane fetches the
+// chunk that belongs there.
+__device__ __forceinline__ int chunk_swap(int r) { return ((r >> 2) ^ (r >> 3)) & 1; }
 __device__ __forceinline__ int lds_off(int s, int r, int c) {
-  return s * kPlaneTileBytes + r * 32 + c * 16;
+  return s * kPlaneTileBytes + r * 32 + ((c ^ chunk_swap(r)) << 4);
 }
 
 // ----------------------------------------------------------------- split
@@ -220,16 +225,21 @@ struct SplitGemm {
   // (A plain member taking the stage's arrays: a device builtin inside a
   // member template fails clang's host-side pass.)
   __device__ __forceinline__ void issue(int kt, char *la, char *lb) {
+    // Lane l lands at LDS position l of the block (row l / 2, physical chunk
+    // l % 2) and fetches the logical chunk stored there (lds_off); blocks
+    // start at multiples of 32 rows, so the row within the block decides.
+    const int row = lane >> 1;
+    const int src = row * 32 + (((lane & 1) ^ chunk_swap(row)) << 4);
 #pragma unroll
     for (int j = 0; j < kPerWave; ++j)
       __builtin_amdgcn_raw_ptr_buffer_load_lds(block_rsrc(ga[j] + kt * sa),
                                                to_lds(la + (wave * kPerWave + j) * 1024), 16,
-                                               lane * 16, 0, 0, 0);
+                                               src, 0, 0, 0);
 #pragma unroll
     for (int j = 0; j < kPerWave; ++j)
       __builtin_amdgcn_raw_ptr_buffer_load_lds(block_rsrc(gb[j] + kt * sb),
                                                to_lds(lb + (wave * kPerWave + j) * 1024), 16,
-                                               lane * 16, 0, 0, 0);
+                                               src, 0, 0, 0);
   }
 
 
