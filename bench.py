@@ -25,7 +25,8 @@ synthetic data of the stated shape:
     per-call trsm;
   - serving: 10k requests (60% gemm, 25% trsm, 15% complex gemm), Poisson
     at 70% of the measured closed-loop capacity, 200 GB admission cap:
-    p50/p99 latency and TFLOP/s, continuous batching vs plain per-call;
+    p50/p99 latency and TFLOP/s, continuous batching vs plain per-call on
+    the same trace, and plain per-call at 70% of its own capacity;
   - DGEMM and SGEMM 8192^3 in all 8 Layout x Transpose variants;
   - the small SGEMM (col-major NN 256^3, U[-1,1] seed 0, beta 0.5): device
     time per call and host-slice call time;
@@ -532,11 +533,34 @@ def serving(count):
     batched = serve.summary(res, st)
     res2, st2 = serve.run(trace, serve.PLAIN)
     plain = serve.summary(res2, st2)
+    # The plain per-call path cannot sustain the batched loop's rate (its
+    # queue grows without bound there); it is also run at 70% of its own
+    # closed-loop capacity, a rate it sustains, for a p99-vs-p99 comparison.
+    _, stc = serve.run(closed, serve.PLAIN)
+    plain_capacity = count / stc["wall_s"]
+    plain_rate = 0.7 * plain_capacity
+    res3, st3 = serve.run(serve.workload(count, plain_rate), serve.PLAIN)
+    plain_own = serve.summary(res3, st3)
     keys = ("tflops", "p50_ms", "p99_ms", "wall_s", "batches", "peak_footprint_gb")
     return {"requests": count, "rate_rps": round(rate, 1),
             "capacity_rps": round(capacity, 1),
             "batched": {k: round(batched[k], 3) for k in keys},
-            "plain": {k: round(plain[k], 3) for k in keys}}, trace
+            "plain": {k: round(plain[k], 3) for k in keys},
+            "plain_capacity_rps": round(plain_capacity, 1),
+            "plain_at_own_70pct": dict({"rate_rps": round(plain_rate, 1)},
+                                       **{k: round(plain_own[k], 3) for k in keys})}, trace
+
+
+def serving_batched(count):
+    """The continuous-batching runs of serving() alone (closed loop, then
+    Poisson at 70% of its capacity), for profiling the batched kernel mix."""
+    from rblas_amd import serve
+    res, st = serve.run(serve.workload(count, 0.0), serve.BATCHED)
+    rate = 0.7 * count / st["wall_s"]
+    res, st = serve.run(serve.workload(count, rate), serve.BATCHED)
+    out = serve.summary(res, st)
+    out["rate_rps"] = rate
+    return {k: round(v, 3) for k, v in out.items()}
 
 
 def c64_gemm(n=4096):
@@ -704,6 +728,7 @@ ONLY = {
     "ctrsm": lambda a: ctrsm_single(),
     "level2": lambda a: level2_bw(),
     "serve": lambda a: serving(a.serve_requests)[0],
+    "serve_batched": lambda a: serving_batched(a.serve_requests),
     "c64_gemm": lambda a: c64_gemm(),
     "sgemm_skinny": lambda a: skinny_sgemm_tflops(),
     "level1": lambda a: level1_bw(),

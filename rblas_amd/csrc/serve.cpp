@@ -11,9 +11,11 @@
 //   3. while a stream slot is free and the queue is not empty, the queue's
 //      head requests are admitted in order as long as the admitted footprint
 //      stays under the cap (and the batch under max_batch), and launched as
-//      one batch on that slot: one generation launch, one grouped gemm launch
-//      per dtype, one grouped trsm launch, 4M launches per complex request,
-//      one hash launch, stream-ordered frees, an event.
+//      one batch on that slot: one generation launch, one launch splitting
+//      every complex request into its 4M real planes, one grouped gemm
+//      launch per dtype (the complex requests' two real products join the
+//      f64 one), one grouped trsm launch per dtype, one launch combining the
+//      complex results, one hash launch, an event.
 // Requests arriving while batches run are therefore admitted as slots and
 // memory free up, and their batch overlaps the running ones on the device.
 // The plain mode (the per-call comparator) serves one request at a time in
@@ -166,6 +168,8 @@ struct Planner {
     std::vector<int2> gen_units;
     std::vector<HashDesc> hashes;
     std::vector<int2> hash_units;
+    std::vector<CplxDesc> cplx;  // batched complex requests
+    std::vector<int2> cplx_split_units, cplx_comb_units;
     const Tuning tu = tuning_snapshot();
     GemmGroup<float> g32;
     GemmGroup<double> g64;
@@ -222,6 +226,29 @@ struct Planner {
         add_gen(b, lb, 1, 0, 0, s.seed, la);
         add_gen(c, lc, 1, 0, 0, s.seed, la + lb);
         add_hash(c, 2 * (int64_t)s.m, s.n, 8);
+        if (!plain) {
+          // 4M: the planes in one batched split launch, the two real
+          // products m x n x 2k in the batch's grouped f64 launch, C in one
+          // batched combine launch.
+          const int64_t m = s.m, n = s.n, k = s.k;
+          double *ws = (double *)(ops + r.off[3]);
+          double *ap = ws, *b1 = ap + 2 * m * k, *b2 = b1 + 2 * k * n, *pr = b2 + 2 * k * n;
+          double *pi = pr + m * n;
+          const int d = (int)cplx.size();
+          cplx.push_back(CplxDesc{(const double *)a, (const double *)b, (double *)c, ws, m, n, k,
+                                  1.0, 0.0, 0.5, 0.0});
+          const int64_t chunks = (m * k + kGenChunk - 1) / kGenChunk +
+                                 (k * n + kGenChunk - 1) / kGenChunk;
+          for (int64_t ch = 0; ch < chunks; ++ch) cplx_split_units.push_back(int2{d, (int)ch});
+          for (int64_t ch = 0; ch * kGenChunk < m * n; ++ch)
+            cplx_comb_units.push_back(int2{d, (int)ch});
+          g64.add(GemmProblem<double>{false, false, m, n, 2 * k, 1.0, 0.0, ap, m, b1, 2 * k, pr,
+                                      m},
+                  tu);
+          g64.add(GemmProblem<double>{false, false, m, n, 2 * k, 1.0, 0.0, ap, m, b2, 2 * k, pi,
+                                      m},
+                  tu);
+        }
       }
     }
     g32.finish();
@@ -243,6 +270,9 @@ struct Planner {
     const size_t o_hd = take(hashes.size() * sizeof(HashDesc));
     const size_t o_hu = take(hash_units.size() * sizeof(int2));
     const size_t o_ho = take(hashes.size() * sizeof(uint64_t));
+    const size_t o_cd = take(cplx.size() * sizeof(CplxDesc));
+    const size_t o_cs = take(cplx_split_units.size() * sizeof(int2));
+    const size_t o_cc = take(cplx_comb_units.size() * sizeof(int2));
     const size_t o_g32 = take(g32.units.empty() ? 0 : g32.staged_bytes());
     const size_t o_g64 = take(g64.units.empty() ? 0 : g64.staged_bytes());
     const size_t o_t32 = take(t32.units.empty() ? 0 : t32.staged_bytes());
@@ -263,12 +293,16 @@ struct Planner {
     for (auto &x : t64.descs) x.a = rebase(x.a), x.b = rebase(x.b);
     for (auto &x : gens) x.ptr = rebase((char *)x.ptr);
     for (auto &x : hashes) x.ptr = rebase((const char *)x.ptr);
+    for (auto &x : cplx) x.a = rebase(x.a), x.b = rebase(x.b), x.c = rebase(x.c), x.ws = rebase(x.ws);
     char *h = slot.pinned;
     std::memcpy(h + o_gd, gens.data(), gens.size() * sizeof(GenDesc));
     std::memcpy(h + o_gu, gen_units.data(), gen_units.size() * sizeof(int2));
     std::memcpy(h + o_hd, hashes.data(), hashes.size() * sizeof(HashDesc));
     std::memcpy(h + o_hu, hash_units.data(), hash_units.size() * sizeof(int2));
     std::memset(h + o_ho, 0, hashes.size() * sizeof(uint64_t));
+    std::memcpy(h + o_cd, cplx.data(), cplx.size() * sizeof(CplxDesc));
+    std::memcpy(h + o_cs, cplx_split_units.data(), cplx_split_units.size() * sizeof(int2));
+    std::memcpy(h + o_cc, cplx_comb_units.data(), cplx_comb_units.size() * sizeof(int2));
     if (!g32.units.empty()) g32.stage(h + o_g32);
     if (!g64.units.empty()) g64.stage(h + o_g64);
     if (!t32.units.empty()) t32.stage(h + o_t32);
@@ -279,6 +313,9 @@ struct Planner {
     if (e != hipSuccess) return e;
     e = launch_gen((const GenDesc *)(d + o_gd), (const int2 *)(d + o_gu), (int)gen_units.size(),
                    st);
+    if (e != hipSuccess) return e;
+    e = launch_cplx_split((const CplxDesc *)(d + o_cd), (const int2 *)(d + o_cs),
+                          (int)cplx_split_units.size(), st);
     if (e != hipSuccess) return e;
     if (!plain) {
       if (!g32.units.empty() &&
@@ -298,7 +335,7 @@ struct Planner {
       const Req &r = reqs[ids[i]];
       const rblas_serve_request_t &s = r.spec;
       char *a = ops + r.off[0], *b = ops + r.off[1], *c = ops + r.off[2];
-      if (s.kind == RBLAS_SERVE_ZGEMM) {
+      if (plain && s.kind == RBLAS_SERVE_ZGEMM) {
         CgemmProblem<double> p{0, 0, s.m, s.n, s.k, 1.0, 0.0, 0.5, 0.0, (const double *)a, s.m,
                                (const double *)b, s.k, (double *)c, s.m};
         e = launch_cgemm<double>(p, (double *)(ops + r.off[3]), st);
@@ -324,6 +361,9 @@ struct Planner {
       }
       if (e != hipSuccess) return e;
     }
+    e = launch_cplx_combine((const CplxDesc *)(d + o_cd), (const int2 *)(d + o_cc),
+                            (int)cplx_comb_units.size(), st);
+    if (e != hipSuccess) return e;
     e = launch_hash((const HashDesc *)(d + o_hd), (const int2 *)(d + o_hu), (int)hash_units.size(),
                     (uint64_t *)(d + o_ho), st);
     if (e != hipSuccess) return e;

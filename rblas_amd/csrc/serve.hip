@@ -40,22 +40,42 @@ __device__ __forceinline__ double uniform_at<double>(uint64_t seed, int64_t idx)
   return 2.0 * u - 1.0;
 }
 
+// Row i and column j of flat index e = i + j * rows, for the first index a
+// thread visits; later indices advance (i, j) by the block stride without a
+// division (a 64-bit division is a long instruction sequence on the GPU).
+__device__ __forceinline__ void split_index(int64_t e, int64_t rows, int64_t &i, int64_t &j) {
+  j = e / rows;
+  i = e - j * rows;
+}
+__device__ __forceinline__ void advance(int64_t step, int64_t rows, int64_t &i, int64_t &j) {
+  i += step;
+  while (i >= rows) {
+    i -= rows;
+    ++j;
+  }
+}
+
 template <typename T>
 __device__ __forceinline__ void gen_range(const GenDesc &d, int64_t lo, int64_t hi) {
   T *p = (T *)d.ptr;
-  for (int64_t e = lo + threadIdx.x; e < hi; e += blockDim.x) {
+  const int64_t e0 = lo + threadIdx.x;
+  if (d.mode == 0) {
+    for (int64_t e = e0; e < hi; e += blockDim.x) p[e] = uniform_at<T>(d.seed, d.offset + e);
+    return;
+  }
+  int64_t i, j;
+  split_index(e0, d.n, i, j);
+  for (int64_t e = e0; e < hi; e += blockDim.x) {
     const T u = uniform_at<T>(d.seed, d.offset + e);
-    T v = u;
-    if (d.mode == 1) {
-      const int64_t i = e % d.n, j = e / d.n;
-      if (i > j)
-        v = (T)((double)u / __builtin_sqrt((double)d.n));
-      else if (i == j)
-        v = (T)(2.0 + __builtin_fabs((double)u));
-      else
-        v = T(0);
-    }
+    T v;
+    if (i > j)
+      v = (T)((double)u / __builtin_sqrt((double)d.n));
+    else if (i == j)
+      v = (T)(2.0 + __builtin_fabs((double)u));
+    else
+      v = T(0);
     p[e] = v;
+    advance(blockDim.x, d.n, i, j);
   }
 }
 
@@ -79,15 +99,28 @@ __global__ __launch_bounds__(256) void hash_kernel(const HashDesc *descs, const 
   const int64_t lo = (int64_t)u.y * kGenChunk;
   const int64_t hi = min(total, lo + (int64_t)kGenChunk);
   unsigned long long acc = 0;
-  for (int64_t e = lo + threadIdx.x; e < hi; e += blockDim.x) {
-    const int64_t i = e % d.rows, j = e / d.rows;
-    const int64_t at = i + j * d.ld;
-    uint64_t bits;
-    if (d.elem_bytes == 4)
-      bits = ((const uint32_t *)d.ptr)[at];
-    else
-      bits = ((const uint64_t *)d.ptr)[at];
-    acc += mix(bits ^ ((uint64_t)e * kGolden));
+  const int64_t e0 = lo + threadIdx.x;
+  if (d.ld == d.rows) {  // dense block: the element of index e is at e
+    if (d.elem_bytes == 4) {
+      const uint32_t *x = (const uint32_t *)d.ptr;
+      for (int64_t e = e0; e < hi; e += blockDim.x) acc += mix(x[e] ^ ((uint64_t)e * kGolden));
+    } else {
+      const uint64_t *x = (const uint64_t *)d.ptr;
+      for (int64_t e = e0; e < hi; e += blockDim.x) acc += mix(x[e] ^ ((uint64_t)e * kGolden));
+    }
+  } else {
+    int64_t i, j;
+    split_index(e0, d.rows, i, j);
+    for (int64_t e = e0; e < hi; e += blockDim.x) {
+      const int64_t at = i + j * d.ld;
+      uint64_t bits;
+      if (d.elem_bytes == 4)
+        bits = ((const uint32_t *)d.ptr)[at];
+      else
+        bits = ((const uint64_t *)d.ptr)[at];
+      acc += mix(bits ^ ((uint64_t)e * kGolden));
+      advance(blockDim.x, d.rows, i, j);
+    }
   }
   part[threadIdx.x] = acc;
   __syncthreads();
@@ -98,7 +131,88 @@ __global__ __launch_bounds__(256) void hash_kernel(const HashDesc *descs, const 
   if (threadIdx.x == 0) atomicAdd(&out[u.x], part[0]);
 }
 
+// Complex requests (complex3.rs:27-74 as 4M, NoTrans / NoTrans, dense
+// column-major operands): the real planes of every complex request of a
+// batch in one launch, before the batch's grouped real gemms compute
+// P_re = [Ar | Ai][Br; -Bi] and P_im = [Ar | Ai][Bi; Br]; then one launch
+// applies C = alpha P + (beta == 0 ? 0 : beta C) with rblas's complex
+// multiply (complex.rs:101-111) to every request. Units are {request, chunk
+// of kGenChunk elements}; a split request's chunks cover op(A)'s m k
+// entries first, then op(B)'s k n.
+__device__ __forceinline__ int64_t a_chunks(const CplxDesc &d) {
+  return (d.m * d.k + kGenChunk - 1) / kGenChunk;
+}
+
+__global__ __launch_bounds__(256) void cplx_split_kernel(const CplxDesc *descs,
+                                                         const int2 *units) {
+  const int2 u = units[blockIdx.x];
+  const CplxDesc d = descs[u.x];
+  const int64_t mk = d.m * d.k, kn = d.k * d.n;
+  double *ap = d.ws, *b1 = ap + 2 * mk, *b2 = b1 + 2 * kn;
+  const int64_t ca = a_chunks(d);
+  if (u.y < ca) {  // [Ar | Ai]: entry e = i + p m of op(A)
+    const int64_t lo = (int64_t)u.y * kGenChunk, hi = min(mk, lo + (int64_t)kGenChunk);
+    for (int64_t e = lo + threadIdx.x; e < hi; e += blockDim.x) {
+      ap[e] = d.a[2 * e];
+      ap[e + mk] = d.a[2 * e + 1];
+    }
+    return;
+  }
+  // [Br; -Bi] and [Bi; Br] (2k x n, ld 2k): entry e = p + j k of op(B)
+  const int64_t lo = (u.y - ca) * kGenChunk, hi = min(kn, lo + (int64_t)kGenChunk);
+  const int64_t e0 = lo + threadIdx.x;
+  int64_t p, j;
+  split_index(e0, d.k, p, j);
+  for (int64_t e = e0; e < hi; e += blockDim.x) {
+    const double re = d.b[2 * e], im = d.b[2 * e + 1];
+    const int64_t col = j * 2 * d.k;
+    b1[col + p] = re;
+    b1[col + d.k + p] = -im;
+    b2[col + p] = im;
+    b2[col + d.k + p] = re;
+    advance(blockDim.x, d.k, p, j);
+  }
+}
+
+__global__ __launch_bounds__(256) void cplx_combine_kernel(const CplxDesc *descs,
+                                                           const int2 *units) {
+  const int2 u = units[blockIdx.x];
+  const CplxDesc d = descs[u.x];
+  const int64_t mk = d.m * d.k, kn = d.k * d.n, mn = d.m * d.n;
+  const double *pr = d.ws + 2 * mk + 4 * kn, *pi = pr + mn;
+  const bool beta_zero = d.beta_re == 0.0 && d.beta_im == 0.0;
+  const int64_t lo = (int64_t)u.y * kGenChunk, hi = min(mn, lo + (int64_t)kGenChunk);
+  for (int64_t e = lo + threadIdx.x; e < hi; e += blockDim.x) {
+    const double p_re = pr[e], p_im = pi[e];
+    double s_re = 0.0, s_im = 0.0;
+    double *cp = d.c + 2 * e;
+    if (!beta_zero) {
+      const double c_re = cp[0], c_im = cp[1];
+      s_re = __builtin_fma(d.beta_re, c_re, -(d.beta_im * c_im));
+      s_im = __builtin_fma(d.beta_re, c_im, d.beta_im * c_re);
+    }
+    const double q_re = __builtin_fma(d.alpha_re, p_re, -(d.alpha_im * p_im));
+    const double q_im = __builtin_fma(d.alpha_re, p_im, d.alpha_im * p_re);
+    cp[0] = q_re + s_re;
+    cp[1] = q_im + s_im;
+  }
+}
+
 }  // namespace
+
+hipError_t launch_cplx_split(const CplxDesc *descs, const int2 *units, int nunits,
+                             hipStream_t st) {
+  if (nunits <= 0) return hipSuccess;
+  hipLaunchKernelGGL(cplx_split_kernel, dim3((unsigned)nunits), dim3(256), 0, st, descs, units);
+  return hipGetLastError();
+}
+
+hipError_t launch_cplx_combine(const CplxDesc *descs, const int2 *units, int nunits,
+                               hipStream_t st) {
+  if (nunits <= 0) return hipSuccess;
+  hipLaunchKernelGGL(cplx_combine_kernel, dim3((unsigned)nunits), dim3(256), 0, st, descs, units);
+  return hipGetLastError();
+}
 
 hipError_t launch_gen(const GenDesc *descs, const int2 *units, int nunits, hipStream_t st) {
   if (nunits <= 0) return hipSuccess;
