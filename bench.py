@@ -610,7 +610,7 @@ def cpu_baseline(n, ld):
     operands, on the host cores: median of 3 calls after a warm-up call on a
     64-column slice. The former 512-column slice is kept as an extra field:
     its 12-column bands re-pack A far more often per flop than the full
-    problem's 128-column bands, so it understates rblas."""
+    problem's 132-column bands, so it understates rblas."""
     from oracle import oracle as orc
     from rblas_amd import workloads
     threads = min(os.cpu_count() or 1, 16)
