@@ -54,13 +54,35 @@ struct TrsmDesc {
   int q, nrhs;
   int trans, upper, unit, right;
   int forward;  // substitution from row 0 up (lower triangle in Left form)
-  int width;    // right-hand sides per work unit: 16 (q <= 1024) or 8 (q <= 2048)
+  int width;    // right-hand sides per work unit (trsm_unit_width)
   T alpha;
   int64_t inv0;  // element offset of this problem's diagonal-block inverses
 };
 
 // q * width must fit the unit's LDS (f64).
 constexpr int kTrsmLdsDoubles = 16384;
+
+// Work-unit queues of a grouped trsm launch, one per XCD (workgroup i runs on
+// XCD i % 8). The launch's sched block (kTrsmSchedBytes) holds the claim
+// counter of queue x at int x * kTrsmLine and the exit counter at int
+// kTrsmQueues * kTrsmLine -- each on a 128-byte line of its own, so claims on
+// different queues do not queue up behind one another -- and, from int
+// kTrsmQueueStart, the queues' unit offsets (kTrsmQueues + 1 ints, staged by
+// the host).
+constexpr int kTrsmQueues = 8;
+constexpr int kTrsmLine = 32;
+constexpr int kTrsmQueueStart = (kTrsmQueues + 1) * kTrsmLine;
+constexpr int kTrsmSchedBytes = 1280;
+static_assert((kTrsmQueueStart + kTrsmQueues + 1) * 4 <= kTrsmSchedBytes, "sched block");
+
+// Right-hand sides per trsm work unit: 16 for orders up to 1024 (held in
+// registers; its op(A) row loads use 32-bit buffer offsets up to
+// (63 lda + 15) elements), else 8 (orders up to 2048, held in LDS).
+template <typename T>
+inline int trsm_unit_width(int64_t q, int64_t lda) {
+  const bool fits = lda * 63 + 15 < ((int64_t)1 << 31) / (int64_t)sizeof(T);
+  return q <= kTrsmLdsDoubles / 16 && fits ? 16 : 8;
+}
 
 template <typename T>
 int grouped_trsm_grid();
@@ -78,7 +100,8 @@ hipError_t launch_trsm_invert(const TrsmDesc<T> &d, double *inv, hipStream_t str
 
 // Inverts the diagonal blocks listed in `blocks` ({problem, block}), then
 // runs one persistent launch over `units` ({problem, first right-hand
-// side}). sched[0..1] as for launch_grouped_gemm; `inv` holds 4096 f64 per
+// side}) in kTrsmQueues queues (sched: counters zero, queue offsets staged at
+// kTrsmQueueStart; reset by the launch itself). `inv` holds 4096 f64 per
 // diagonal block.
 template <typename T>
 hipError_t launch_grouped_trsm(const TrsmDesc<T> *descs, const int2 *blocks, int nblocks,

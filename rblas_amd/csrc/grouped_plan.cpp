@@ -187,16 +187,20 @@ void TrsmGroup<T>::add(bool left, bool trans, bool eff_upper, bool unit, int64_t
   d.unit = unit;
   d.right = !left;
   d.forward = !d.upper;
-  d.width = d.q <= kTrsmLdsDoubles / 16 ? 16 : 8;
+  d.width = trsm_unit_width<T>(d.q, d.lda);
   d.alpha = alpha;
   d.inv0 = inv_elems;
   const int prob = (int)descs.size();
   const int nblk = (d.q + 63) / 64;
   inv_elems += (int64_t)nblk * 64 * 64;
   for (int blk = 0; blk < nblk; ++blk) blocks.push_back(int2{prob, blk});
+  // Unit time model (us of one CU, f64, measured on 16-wide units): a fixed
+  // 3 us, 3.5 us per 64-row diagonal block (its barrier-separated steps), and
+  // the update at about 0.2 TFLOP/s per CU.
+  const double unit_us = 3.0 + 3.5 * nblk + (double)d.q * (double)d.q * d.width / 2.0e5;
   for (int c0 = 0; c0 < d.nrhs; c0 += d.width) {
     units.push_back(int2{prob, c0});
-    cost.push_back((double)d.q * (double)d.q * d.width);
+    cost.push_back(unit_us);
   }
   info.flops += (double)d.q * (double)d.q * (double)d.nrhs;
   descs.push_back(d);
@@ -204,7 +208,38 @@ void TrsmGroup<T>::add(bool left, bool trans, bool eff_upper, bool unit, int64_t
 
 template <typename T>
 void TrsmGroup<T>::finish() {
-  order_longest_first(units, cost);
+  // add() appends each problem's units consecutively, all of one cost.
+  struct Chunk {
+    size_t u0, n;
+    double unit_cost;
+  };
+  constexpr size_t kChunk = kCUs / kTrsmQueues;
+  std::vector<Chunk> chunks;
+  for (size_t u = 0; u < units.size();) {
+    size_t e = u;
+    while (e < units.size() && units[e].x == units[u].x) ++e;
+    for (size_t c = u; c < e; c += kChunk) chunks.push_back(Chunk{c, std::min(kChunk, e - c), cost[u]});
+    u = e;
+  }
+  std::stable_sort(chunks.begin(), chunks.end(),
+                   [](const Chunk &x, const Chunk &y) { return x.unit_cost > y.unit_cost; });
+  std::vector<std::vector<int2>> q(kTrsmQueues);
+  double load[kTrsmQueues] = {};
+  for (const Chunk &c : chunks) {
+    int best = 0;
+    for (int x = 1; x < kTrsmQueues; ++x)
+      if (load[x] < load[best]) best = x;
+    load[best] += c.unit_cost * (double)c.n;
+    q[best].insert(q[best].end(), units.begin() + c.u0, units.begin() + c.u0 + c.n);
+  }
+  std::vector<int2> sorted;
+  sorted.reserve(units.size());
+  for (int x = 0; x < kTrsmQueues; ++x) {
+    qstart[x] = (int)sorted.size();
+    sorted.insert(sorted.end(), q[x].begin(), q[x].end());
+  }
+  qstart[kTrsmQueues] = (int)sorted.size();
+  units.swap(sorted);
   info.units = units.size();
 }
 
@@ -214,6 +249,7 @@ void TrsmGroup<T>::stage(char *h) const {
   std::memcpy(h, descs.data(), descs.size() * sizeof(TrsmDesc<T>));
   std::memcpy(h + off_units(), units.data(), units.size() * sizeof(int2));
   std::memcpy(h + off_blocks(), blocks.data(), blocks.size() * sizeof(int2));
+  std::memcpy(h + off_sched() + kTrsmQueueStart * sizeof(int), qstart, sizeof(qstart));
 }
 
 template <typename T>

@@ -62,18 +62,22 @@ struct TrsmGroup {
   std::vector<int2> blocks, units;
   std::vector<double> cost;
   int64_t inv_elems = 0;
+  int qstart[kTrsmQueues + 1] = {};  // units of queue x: [qstart[x], qstart[x + 1])
   rblas_grouped_info_t info{};
 
   // One problem with m, n > 0 and the order of A at most 2048, as rblas's
   // trsm arguments (stored A, column-major B).
   void add(bool left, bool trans, bool eff_upper, bool unit, int64_t m, int64_t n, T alpha,
            const T *a, int64_t lda, T *b, int64_t ldb);
+  // Splits the units into one queue per XCD: a system's units stay together
+  // (up to 32, one XCD's CUs, per chunk) so that they share op(A) through
+  // that XCD's L2; chunks go longest unit first to the least loaded queue.
   void finish();
 
   size_t off_units() const { return align256(descs.size() * sizeof(TrsmDesc<T>)); }
   size_t off_blocks() const { return off_units() + align256(units.size() * sizeof(int2)); }
   size_t off_sched() const { return off_blocks() + align256(blocks.size() * sizeof(int2)); }
-  size_t off_ws() const { return off_sched() + 256; }
+  size_t off_ws() const { return off_sched() + kTrsmSchedBytes; }
   size_t staged_bytes() const { return off_ws(); }
   size_t bytes() const { return off_ws() + (size_t)inv_elems * sizeof(double); }
   size_t ws_bytes() const { return (size_t)inv_elems * sizeof(double); }

@@ -6,8 +6,9 @@
 // op(A)^T X^T = alpha B^T, so a Right problem becomes Left with op(A)
 // transposed (trans flipped, triangle flipped) on B^T. A work unit is one
 // system and W consecutive right-hand sides (columns of B for Left, rows of
-// B for Right). The unit keeps its W right-hand sides in LDS as f64
-// (q x W, q the order of A, q * W <= 16384), then walks the diagonal blocks
+// B for Right). The unit keeps its W right-hand sides as f64 -- 16 of them
+// for orders up to 1024 in the waves' MFMA accumulators (solve_unit_reg), 8
+// for orders up to 2048 in LDS (solve_unit) -- then walks the diagonal blocks
 // in substitution order:
 //   X_b = inv_b * B_b   (f64 MFMA over the 16 x 16 sub-blocks of inv_b; a
 //                        diagonal sub-block whose B rows hold a non-finite
@@ -340,7 +341,11 @@ __device__ __forceinline__ void update_rows(double *Bs, const T (&pv)[16], const
 template <typename T, int W>
 __device__ void solve_unit(const TrsmDesc<T> &d, int c0, const double *inv, double *Bs,
                            double *Is) {
-  const int t = threadIdx.x, lane = t & 63, wave = uni(t >> 6);
+  // Opaque copy of the thread id: in the persistent loop every per-lane
+  // offset is recomputed per unit instead of hoisted out of it (and spilled).
+  int t = threadIdx.x;
+  asm volatile("" : "+v"(t));
+  const int lane = t & 63, wave = uni(t >> 6);
   const int q = d.q;
   const int nc = min(W, d.nrhs - c0);
   const int qpad = (q + kNB - 1) / kNB * kNB;
@@ -506,33 +511,374 @@ __device__ void solve_unit(const TrsmDesc<T> &d, int c0, const double *inv, doub
   }
 }
 
+// Buffer loads for solve_unit_reg's op(A) rows: one 32-bit lane offset and a
+// scalar step per k, instead of a 64-bit address per load. An offset of
+// kPast lies beyond the resource's num_records (clamped to 2^31 - 1), which
+// the hardware reads as 0. In-block offsets stay below 2^31 for the leading
+// dimensions trsm_unit_width gives width 16.
+constexpr uint32_t kPast = 0x80000000u;
+template <typename T>
+__device__ __forceinline__ T buf_load(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff);
+template <>
+__device__ __forceinline__ double buf_load<double>(__amdgpu_buffer_rsrc_t r, uint32_t voff,
+                                                   uint32_t soff) {
+  return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+}
+template <>
+__device__ __forceinline__ float buf_load<float>(__amdgpu_buffer_rsrc_t r, uint32_t voff,
+                                                 uint32_t soff) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
+}
+
+// op(A)(rb + l%16, r0 + 4t + l/16), t = 0..15, for lane l; entries outside
+// rows [.., hi) or block columns [0, nb) read as 0.
+template <typename T>
+__device__ __forceinline__ void load_rows_buf(const TrsmDesc<T> &d, int rb, int hi, int r0, int nb,
+                                              int64_t si, int64_t sj, int lane, T (&v)[16]) {
+  const int kq = lane >> 4;
+  const T *o = d.a + (int64_t)rb * si + (int64_t)r0 * sj;
+  const __amdgpu_buffer_rsrc_t rs =
+      __builtin_amdgcn_make_buffer_rsrc(const_cast<T *>(o), (short)0, 0x7fffffff, 0x00020000);
+  const bool row_ok = rb + (lane & 15) < hi;
+  const uint32_t base = row_ok ? (uint32_t)(((lane & 15) * si + kq * sj) * (int64_t)sizeof(T)) : kPast;
+  const uint32_t step = (uint32_t)(4 * sj * (int64_t)sizeof(T));
+  if (nb == kNB) {
+#pragma unroll
+    for (int t4 = 0; t4 < 16; ++t4) v[t4] = buf_load<T>(rs, base, (uint32_t)t4 * step);
+  } else {
+#pragma unroll
+    for (int t4 = 0; t4 < 16; ++t4)
+      v[t4] = buf_load<T>(rs, 4 * t4 + kq < nb ? base : kPast, (uint32_t)t4 * step);
+  }
+}
+
+typedef __attribute__((address_space(3))) void *lds_void_ptr;
+
+// Workgroup barrier that leaves this wave's last N vector-memory loads in
+// flight and publishes its LDS stores: vmcnt(N) lgkmcnt(0), then s_barrier.
+// Loads retire in order, so everything issued before those N (an LDS DMA
+// included) has landed. __syncthreads() would drain every load instead.
+template <int N>
+__device__ __forceinline__ void barrier_vm() {
+  static_assert(N >= 0 && N < 64, "vmcnt range");
+  asm volatile("" ::: "memory");
+  __builtin_amdgcn_s_waitcnt((N & 0xf) | ((N >> 4) << 14) | 0x0070);
+  __builtin_amdgcn_s_barrier();
+  asm volatile("" ::: "memory");
+}
+
+// acc += (-X_b) op(A)[rb .. rb+15, block]^T in the transposed accumulator
+// layout of solve_unit_reg, from the loads of load_rows_buf.
+template <typename T>
+__device__ __forceinline__ f64x4 update_acc(f64x4 acc, const T (&pv)[16],
+                                            const double (&xq)[16]) {
+#pragma unroll
+  for (int t4 = 0; t4 < 16; ++t4) acc = mma64(xq[t4], (double)pv[t4], acc);
+  return acc;
+}
+
+// A diagonal-block inverse (64 x 64 f64 at ib) into Is in the sub-block
+// order of inv_src, by LDS DMA: 16 bytes a lane, two consecutive entries of
+// one sub-block column. Completes with the wave's vector-memory count.
+__device__ __forceinline__ void dma_inverse(const double *ib, bool upper, double *Is, int wave,
+                                            int lane) {
+  const __amdgpu_buffer_rsrc_t ir = __builtin_amdgcn_make_buffer_rsrc(
+      const_cast<double *>(ib), (short)0, kNB * kNB * (int)sizeof(double), 0x00020000);
+#pragma unroll
+  for (int u = 0; u < (kInvLds / 2 + kThreads - 1) / kThreads; ++u) {
+    const int p0 = (u * kWaves + wave) * 64;  // first entry pair of this wave's share
+    if (p0 < kInvLds / 2)
+      __builtin_amdgcn_raw_ptr_buffer_load_lds(
+          ir, (lds_void_ptr)(Is + 2 * p0), 16,
+          (uint32_t)inv_src(upper, 2 * (p0 + lane)) * (uint32_t)sizeof(double), 0, 0, 0);
+  }
+}
+
+// Width-16 units (orders up to 16 * kWaves * RB) with the right-hand sides
+// held in registers instead of LDS. 16-row block g of the unit's B belongs to
+// wave g % kWaves as the f64 MFMA accumulator acc[g / kWaves], transposed:
+// element r of lane l is B(16 g + (l & 15), (l >> 4) + 4 r). An update is
+// then acc += (-X_b)^T op(A)[rows, block b]^T -- the X fragments are the
+// product's first operand, the streamed op(A) rows (the same loads as
+// solve_unit) its second -- so B does not pass through LDS at all, apart from
+// the 64 rows of the current diagonal block. Per diagonal block b:
+//   1. the four waves owning b's rows stage them in Bb (rows at or past q as
+//      zeros); every wave issues the op(A) loads of its first row block;
+//   2. the owners form X_b = inv_b B_b from Bb and Is (f64 MFMA; the scalar
+//      triangle for a diagonal sub-block with a non-finite B entry, as in
+//      solve_unit), keep X_b in their accumulators and publish -X_b in Xs;
+//   3. every wave updates its row blocks in the rest of the system, loading
+//      each row block's op(A) one block ahead of its MFMAs;
+//   4. inv_{b+1}, loaded before the update, goes to Is.
+// Two workgroup barriers per diagonal block (after 1 and after 2).
+template <typename T, int RB>
+__device__ void solve_unit_reg(const TrsmDesc<T> &d, int c0, const double *inv, double *Bb,
+                               double *Xs, double *Is) {
+  static_assert(kWaves % 4 == 0, "a diagonal block's four row blocks need four owner waves");
+  constexpr int W = 16;
+  constexpr int kS = 17;  // Bb / Xs row stride in doubles
+  // Opaque copy of the thread id: in the persistent loop every per-lane
+  // offset is recomputed per unit instead of hoisted out of it (and spilled).
+  int t = threadIdx.x;
+  asm volatile("" : "+v"(t));
+  const int lane = t & 63, wave = uni(t >> 6);
+  const int q = d.q;
+  const int nc = min(W, d.nrhs - c0);
+  const int qpad = (q + kNB - 1) / kNB * kNB;
+  const int nrb = qpad / 16;
+  const bool upper = d.upper;
+  const int li = lane & 15, kq = lane >> 4;
+  const int nblk = qpad / kNB;
+  // The first diagonal block's inverse (lands before the first barrier).
+  dma_inverse(inv + d.inv0 + (int64_t)(d.forward ? 0 : nblk - 1) * kNB * kNB, upper, Is, wave,
+              lane);
+  // B, alpha applied to every entry on load (rblas's pre-scale); row blocks
+  // past the order are neither loaded nor stored. Lane addresses advance by
+  // wave-uniform steps: 16 kWaves rows per accumulator, 4 columns per r.
+  const int64_t rstep = d.right ? d.ldb : 1, cstep = d.right ? 1 : d.ldb;
+  const T *bl = d.b + (int64_t)(16 * wave + li) * rstep + (int64_t)(c0 + kq) * cstep;
+  f64x4 acc[RB];
+  {
+    T raw[RB][4];
+#pragma unroll
+    for (int j = 0; j < RB; ++j) {
+      const bool rok = 16 * (wave + kWaves * j) + li < q;
+      if (16 * (wave + kWaves * j) < q) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+          if (rok && kq + 4 * r < nc)
+            raw[j][r] = bl[(int64_t)(16 * kWaves * j) * rstep + (int64_t)(4 * r) * cstep];
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < RB; ++j) {
+      const bool rok = 16 * (wave + kWaves * j) + li < q;
+      if (16 * (wave + kWaves * j) < q) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const T v = d.alpha != T(1) ? d.alpha * raw[j][r] : raw[j][r];
+          acc[j][r] = (rok && kq + 4 * r < nc) ? (double)v : 0.0;
+        }
+      } else {
+        acc[j] = f64x4{0, 0, 0, 0};
+      }
+    }
+  }
+  const int64_t si = d.trans ? d.lda : 1;  // op(A) row step
+  const int64_t sj = d.trans ? 1 : d.lda;  // op(A) column step
+  for (int s = 0; s < nblk; ++s) {
+    const int b = d.forward ? s : nblk - 1 - s;
+    const int r0 = b * kNB;
+    const int nb = min(kNB, q - r0);
+    const int hi = upper ? r0 : q;
+    // Row blocks of the update: [4b + 4, nrb) (lower) or [0, 4b) (upper);
+    // this wave's accumulators among them: j in [ja, jz).
+    const int glo = upper ? 0 : 4 * b + 4, ghi = upper ? 4 * b : nrb;
+    const int ja = glo > wave ? (glo - wave + kWaves - 1) / kWaves : 0;
+    const int jz = ghi > wave ? (ghi - wave + kWaves - 1) / kWaves : 0;
+    // Owners: row block 4b + oi (oi < 4) is wave (4b) % kWaves + oi's
+    // accumulator oj.
+    const int oi = wave - ((4 * b) % kWaves), oj = (4 * b) / kWaves;
+    const bool owner = oi >= 0 && oi < 4;
+    // 1.
+    if (owner) {
+      f64x4 v = acc[0];
+#pragma unroll
+      for (int j = 1; j < RB; ++j)
+        if (j == oj) v = acc[j];
+      const int row = 16 * oi + li;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) Bb[row * kS + kq + 4 * r] = r0 + row < q ? v[r] : 0.0;
+    }
+    T pa[16], pb[16];
+    if (ja < jz) {
+      const int rb = 16 * (wave + kWaves * ja);
+      if (ja & 1)
+        load_rows_buf(d, rb, hi, r0, nb, si, sj, lane, pb);
+      else
+        load_rows_buf(d, rb, hi, r0, nb, si, sj, lane, pa);
+      barrier_vm<16>();  // B_b staged, inv_b landed; the row block stays in flight
+    } else {
+      barrier_vm<0>();
+    }
+    // 2.
+    if (owner) {
+      f64x4 x = {0, 0, 0, 0};
+      if (16 * oi < nb) {
+#pragma unroll
+        for (int jb = 0; jb < 4; ++jb) {
+          const bool use = (upper ? jb > oi : jb < oi) && 16 * jb < nb;  // wave-uniform
+          if (use) {
+            const double *sb = Is + sub_slot(upper, oi, jb);
+            double iq[4], bp[4];
+#pragma unroll
+            for (int t4 = 0; t4 < 4; ++t4) {
+              const int j = 4 * t4 + kq;
+              iq[t4] = sb[li + 16 * j];
+              bp[t4] = Bb[(16 * jb + j) * kS + li];
+            }
+#pragma unroll
+            for (int t4 = 0; t4 < 4; ++t4) x = mma64(bp[t4], iq[t4], x);
+          }
+        }
+        // Diagonal sub-block: the full 16 x 16 product when its B rows are
+        // all finite (the inverse's exact zeros outside the triangle add
+        // exact zeros), else the triangle alone, scalar f64, so a non-finite
+        // entry reaches exactly the X entries rblas's substitution reaches.
+        const double *sd = Is + sub_slot(upper, oi, oi);
+        double pd[4], qd[4];
+        bool fin = true;
+#pragma unroll
+        for (int t4 = 0; t4 < 4; ++t4) {
+          const int j = 4 * t4 + kq;
+          pd[t4] = sd[li + 16 * j];
+          qd[t4] = Bb[(16 * oi + j) * kS + li];
+          fin = fin && __builtin_isfinite(qd[t4]);
+        }
+        if (__all(fin)) {
+#pragma unroll
+          for (int t4 = 0; t4 < 4; ++t4) x = mma64(qd[t4], pd[t4], x);
+        } else {
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const int c = kq + 4 * r;
+            double sum = x[r];
+#pragma unroll
+            for (int jl = 0; jl < 16; ++jl) {
+              const int j = 16 * oi + jl;
+              const bool on = (upper ? jl >= li : jl <= li) && j < nb;
+              const double w = on ? sd[li + 16 * jl] : 0.0;
+              const double xv = on ? Bb[j * kS + c] : 0.0;
+              sum = on ? __builtin_fma(w, xv, sum) : sum;
+            }
+            x[r] = sum;
+          }
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < RB; ++j)
+        if (j == oj) acc[j] = x;
+      const int row = 16 * oi + li;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) Xs[row * kS + kq + 4 * r] = -x[r];
+    }
+    barrier_vm<16>();  // -X_b visible; every read of Is and Bb done
+    // 4. inv_{b+1} straight into Is, landing before the next step's first
+    // barrier.
+    if (s + 1 < nblk)
+      dma_inverse(inv + d.inv0 + (int64_t)(d.forward ? b + 1 : b - 1) * kNB * kNB, upper, Is,
+                  wave, lane);
+    // 3.
+    double xq[16];
+#pragma unroll
+    for (int t4 = 0; t4 < 16; ++t4) {
+      const int j = 4 * t4 + kq;
+      xq[t4] = j < nb ? Xs[j * kS + li] : 0.0;
+    }
+#pragma unroll
+    for (int j = 0; j < RB; ++j) {
+      if (j >= ja && j < jz) {
+        const int rb = 16 * (wave + kWaves * j);
+        if (j & 1) {
+          if (j + 1 < jz) load_rows_buf(d, rb + 16 * kWaves, hi, r0, nb, si, sj, lane, pa);
+          acc[j] = update_acc(acc[j], pb, xq);
+        } else {
+          if (j + 1 < jz) load_rows_buf(d, rb + 16 * kWaves, hi, r0, nb, si, sj, lane, pb);
+          acc[j] = update_acc(acc[j], pa, xq);
+        }
+      }
+    }
+  }
+  T *bo = d.b + (int64_t)(16 * wave + li) * rstep + (int64_t)(c0 + kq) * cstep;
+#pragma unroll
+  for (int j = 0; j < RB; ++j) {
+    const bool rok = 16 * (wave + kWaves * j) + li < q;
+    if (16 * (wave + kWaves * j) < q) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+        if (rok && kq + 4 * r < nc)
+          bo[(int64_t)(16 * kWaves * j) * rstep + (int64_t)(4 * r) * cstep] = (T)acc[j][r];
+    }
+  }
+}
+
+// Width 16 through the register-resident solve, width 8 through the LDS one
+// (trsm_unit_width picks).
+template <typename T>
+__device__ __forceinline__ void solve_any(const TrsmDesc<T> &d, int c0, const double *inv,
+                                          double *Bs, double *Is) {
+  if (d.width == 16)
+    solve_unit_reg<T, kTrsmLdsDoubles / 16 / (16 * kWaves)>(d, c0, inv, Bs, Bs + kNB * 17, Is);
+  else
+    solve_unit<T, 8>(d, c0, inv, Bs, Is);
+}
+
+// Persistent: each workgroup claims units from its XCD's queue (the
+// planner keeps a system's units in one queue, so they meet op(A) in one
+// L2), then helps a queue that has fallen behind. The last workgroup out
+// resets the counters, so the launch can be replayed.
+constexpr int kStealMin = 4;
 template <typename T>
 __global__ __launch_bounds__(kThreads) void grouped_trsm_kernel(const TrsmDesc<T> *descs,
-                                                                const int2 *units, int nunits,
-                                                                int *sched, const double *inv) {
+                                                                const int2 *units, int *sched,
+                                                                const double *inv) {
   __shared__ double Bs[kLdsDoubles];
-  __shared__ double Is[kInvLds];
+  __shared__ __align__(16) double Is[kInvLds];
   __shared__ int s_unit;
+  const int home = (int)(blockIdx.x % kTrsmQueues);
+  // Thread 0 keeps the queue offsets and whether its own queue has run dry.
+  int qs[kTrsmQueues + 1];
+  bool home_open = true;
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int x = 0; x <= kTrsmQueues; ++x) qs[x] = sched[kTrsmQueueStart + x];
+  }
   for (;;) {
-    if (threadIdx.x == 0) s_unit = atomicAdd(&sched[0], 1);
+    if (threadIdx.x == 0) {
+      int u = -1;
+      if (home_open) {
+        const int c = atomicAdd(&sched[home * kTrsmLine], 1);
+        if (c < qs[home + 1] - qs[home])
+          u = qs[home] + c;
+        else
+          home_open = false;
+      }
+      // Own queue drained: read every counter at once and help the queue
+      // furthest behind, if it still holds kStealMin units (fewer are left to
+      // its own workgroups -- every queue has some, the grid being at least
+      // kTrsmQueues -- which keeps the drained tail from stealing units their
+      // owners are about to claim). A lost race re-reads; each loss means
+      // that queue fell below the threshold, so this ends.
+      for (int round = 0; u < 0 && round < kTrsmQueues; ++round) {
+        int left[kTrsmQueues];
+#pragma unroll
+        for (int x = 0; x < kTrsmQueues; ++x)
+          left[x] = qs[x + 1] - qs[x] -
+                    __hip_atomic_load(&sched[x * kTrsmLine], __ATOMIC_RELAXED,
+                                      __HIP_MEMORY_SCOPE_AGENT);
+        int pick = 0;
+#pragma unroll
+        for (int x = 1; x < kTrsmQueues; ++x)
+          if (left[x] > left[pick]) pick = x;
+        if (left[pick] < kStealMin) break;
+        const int c = atomicAdd(&sched[pick * kTrsmLine], 1);
+        if (c < qs[pick + 1] - qs[pick]) u = qs[pick] + c;
+      }
+      s_unit = u;
+    }
     __syncthreads();
     const int u = uni(s_unit);
     __syncthreads();
-    if (u >= nunits) break;
+    if (u < 0) break;
     const int2 w = units[u];
     const TrsmDesc<T> d = descs[uni(w.x)];
-    if (d.width == 16)
-      solve_unit<T, 16>(d, uni(w.y), inv, Bs, Is);
-    else
-      solve_unit<T, 8>(d, uni(w.y), inv, Bs, Is);
+    solve_any(d, uni(w.y), inv, Bs, Is);
     __syncthreads();
   }
   if (threadIdx.x == 0) {
     __threadfence();
-    if (atomicAdd(&sched[1], 1) == (int)gridDim.x - 1) {
-      atomicExch(&sched[0], 0);
-      atomicExch(&sched[1], 0);
-    }
+    if (atomicAdd(&sched[kTrsmQueues * kTrsmLine], 1) == (int)gridDim.x - 1)
+      for (int x = 0; x <= kTrsmQueues; ++x) atomicExch(&sched[x * kTrsmLine], 0);
   }
 }
 
@@ -541,11 +887,8 @@ __global__ __launch_bounds__(kThreads) void grouped_trsm_kernel(const TrsmDesc<T
 template <typename T>
 __global__ __launch_bounds__(kThreads) void single_trsm_solve(TrsmDesc<T> d, const double *inv) {
   __shared__ double Bs[kLdsDoubles];
-  __shared__ double Is[kInvLds];
-  if (d.width == 16)
-    solve_unit<T, 16>(d, (int)blockIdx.x * 16, inv, Bs, Is);
-  else
-    solve_unit<T, 8>(d, (int)blockIdx.x * 8, inv, Bs, Is);
+  __shared__ __align__(16) double Is[kInvLds];
+  solve_any(d, (int)blockIdx.x * d.width, inv, Bs, Is);
 }
 
 }  // namespace
@@ -592,9 +935,12 @@ hipError_t launch_grouped_trsm(const TrsmDesc<T> *descs, const int2 *blocks, int
     hipLaunchKernelGGL((grouped_trsm_inv<T>), dim3((unsigned)nblocks), dim3(kInvThreads), 0, st, descs,
                        blocks, inv);
   if (nunits > 0) {
+    // Every queue needs a workgroup of its own (units below the stealing
+    // threshold are left to it); surplus workgroups find nothing and leave.
     if (grid > nunits) grid = nunits;
+    if (grid < kTrsmQueues) grid = kTrsmQueues;
     hipLaunchKernelGGL((grouped_trsm_kernel<T>), dim3((unsigned)grid), dim3(kThreads), 0, st,
-                       descs, units, nunits, sched, (const double *)inv);
+                       descs, units, sched, (const double *)inv);
   }
   return hipGetLastError();
 }
