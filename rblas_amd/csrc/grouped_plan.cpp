@@ -187,7 +187,7 @@ void TrsmGroup<T>::add(bool left, bool trans, bool eff_upper, bool unit, int64_t
   d.unit = unit;
   d.right = !left;
   d.forward = !d.upper;
-  d.width = trsm_unit_width<T>(d.q, d.lda);
+  d.width = trsm_unit_width<T>(d.q, d.lda, d.nrhs, false);
   d.alpha = alpha;
   d.inv0 = inv_elems;
   const int prob = (int)descs.size();

@@ -339,7 +339,7 @@ __device__ __forceinline__ void update_rows(double *Bs, const T (&pv)[16], const
 //      kWaves rows), loading the next row block before the MFMAs of this one;
 //   5. inv_{b+1} goes to LDS.
 template <typename T, int W>
-__device__ void solve_unit(const TrsmDesc<T> &d, int c0, const double *inv, double *Bs,
+__device__ __forceinline__ void solve_unit(const TrsmDesc<T> &d, int c0, const double *inv, double *Bs,
                            double *Is) {
   // Opaque copy of the thread id: in the persistent loop every per-lane
   // offset is recomputed per unit instead of hoisted out of it (and spilled).
@@ -542,14 +542,9 @@ __device__ __forceinline__ void load_rows_buf(const TrsmDesc<T> &d, int rb, int 
   const bool row_ok = rb + (lane & 15) < hi;
   const uint32_t base = row_ok ? (uint32_t)(((lane & 15) * si + kq * sj) * (int64_t)sizeof(T)) : kPast;
   const uint32_t step = (uint32_t)(4 * sj * (int64_t)sizeof(T));
-  if (nb == kNB) {
 #pragma unroll
-    for (int t4 = 0; t4 < 16; ++t4) v[t4] = buf_load<T>(rs, base, (uint32_t)t4 * step);
-  } else {
-#pragma unroll
-    for (int t4 = 0; t4 < 16; ++t4)
-      v[t4] = buf_load<T>(rs, 4 * t4 + kq < nb ? base : kPast, (uint32_t)t4 * step);
-  }
+  for (int t4 = 0; t4 < 16; ++t4)
+    v[t4] = buf_load<T>(rs, 4 * t4 + kq < nb ? base : kPast, (uint32_t)t4 * step);
 }
 
 typedef __attribute__((address_space(3))) void *lds_void_ptr;
@@ -567,14 +562,46 @@ __device__ __forceinline__ void barrier_vm() {
   asm volatile("" ::: "memory");
 }
 
-// acc += (-X_b) op(A)[rb .. rb+15, block]^T in the transposed accumulator
-// layout of solve_unit_reg, from the loads of load_rows_buf.
+// acc += (-X_b)^T op(A)[rb .. rb+15, block]^T (transposed accumulator layout
+// of solve_unit_reg) from the loads of load_rows_buf and X fragments xq.
 template <typename T>
 __device__ __forceinline__ f64x4 update_acc(f64x4 acc, const T (&pv)[16],
                                             const double (&xq)[16]) {
 #pragma unroll
   for (int t4 = 0; t4 < 16; ++t4) acc = mma64(xq[t4], (double)pv[t4], acc);
   return acc;
+}
+
+// The same over half a block's columns (k in [32 half, 32 half + 32)) for two
+// column blocks, X fragments read from Xs (row stride kS) per product.
+template <typename T>
+__device__ __forceinline__ void update_half(f64x4 (&acc)[2], const T (&pv)[8], const double *Xs,
+                                            int kS, int half, int lane) {
+  const int kq = lane >> 4, li = lane & 15;
+#pragma unroll
+  for (int t = 0; t < 8; ++t) {
+    const double a = (double)pv[t];
+    const double *xr = Xs + (32 * half + 4 * t + kq) * kS + li;
+    acc[0] = mma64(xr[0], a, acc[0]);
+    acc[1] = mma64(xr[16], a, acc[1]);
+  }
+}
+
+// op(A)(rb + l%16, r0 + 32 half + 4t + l/16), t = 0..7, as load_rows_buf.
+template <typename T>
+__device__ __forceinline__ void load_half_buf(const TrsmDesc<T> &d, int rb, int hi, int r0, int nb,
+                                              int64_t si, int64_t sj, int half, int lane,
+                                              T (&v)[8]) {
+  const int kq = lane >> 4;
+  const T *o = d.a + (int64_t)rb * si + (int64_t)(r0 + 32 * half) * sj;
+  const __amdgpu_buffer_rsrc_t rs =
+      __builtin_amdgcn_make_buffer_rsrc(const_cast<T *>(o), (short)0, 0x7fffffff, 0x00020000);
+  const bool row_ok = rb + (lane & 15) < hi;
+  const uint32_t base = row_ok ? (uint32_t)(((lane & 15) * si + kq * sj) * (int64_t)sizeof(T)) : kPast;
+  const uint32_t step = (uint32_t)(4 * sj * (int64_t)sizeof(T));
+#pragma unroll
+  for (int t = 0; t < 8; ++t)
+    v[t] = buf_load<T>(rs, 32 * half + 4 * t + kq < nb ? base : kPast, (uint32_t)t * step);
 }
 
 // A diagonal-block inverse (64 x 64 f64 at ib) into Is in the sub-block
@@ -594,29 +621,145 @@ __device__ __forceinline__ void dma_inverse(const double *ib, bool upper, double
   }
 }
 
-// Width-16 units (orders up to 16 * kWaves * RB) with the right-hand sides
-// held in registers instead of LDS. 16-row block g of the unit's B belongs to
-// wave g % kWaves as the f64 MFMA accumulator acc[g / kWaves], transposed:
-// element r of lane l is B(16 g + (l & 15), (l >> 4) + 4 r). An update is
-// then acc += (-X_b)^T op(A)[rows, block b]^T -- the X fragments are the
-// product's first operand, the streamed op(A) rows (the same loads as
-// solve_unit) its second -- so B does not pass through LDS at all, apart from
-// the 64 rows of the current diagonal block. Per diagonal block b:
-//   1. the four waves owning b's rows stage them in Bb (rows at or past q as
-//      zeros); every wave issues the op(A) loads of its first row block;
-//   2. the owners form X_b = inv_b B_b from Bb and Is (f64 MFMA; the scalar
-//      triangle for a diagonal sub-block with a non-finite B entry, as in
-//      solve_unit), keep X_b in their accumulators and publish -X_b in Xs;
-//   3. every wave updates its row blocks in the rest of the system, loading
-//      each row block's op(A) one block ahead of its MFMAs;
-//   4. inv_{b+1}, loaded before the update, goes to Is.
-// Two workgroup barriers per diagonal block (after 1 and after 2).
-template <typename T, int RB>
-__device__ void solve_unit_reg(const TrsmDesc<T> &d, int c0, const double *inv, double *Bb,
-                               double *Xs, double *Is) {
-  static_assert(kWaves % 4 == 0, "a diagonal block's four row blocks need four owner waves");
-  constexpr int W = 16;
-  constexpr int kS = 17;  // Bb / Xs row stride in doubles
+// Row block of a unit's accumulator j in wave w (substitution order h =
+// w + kWaves j, mirrored for a backward solve).
+__device__ __forceinline__ int row_block(bool upper, int nrb, int w, int j) {
+  return upper ? nrb - 1 - (w + kWaves * j) : w + kWaves * j;
+}
+
+// Rows 16 oi .. 16 oi + 15 of X = inv B for one diagonal block (B staged in
+// Bb, its inverse in Is), transposed accumulator layout: f64 MFMA over the
+// sub-blocks; the diagonal sub-block as a full 16 x 16 product when its B
+// rows are all finite (the inverse's exact zeros outside the triangle add
+// exact zeros), else its triangle alone, scalar f64, so a non-finite entry
+// reaches exactly the X entries rblas's substitution reaches.
+template <int CB>
+__device__ __forceinline__ void form_x(const double *Bb, int kS, const double *Is, bool upper,
+                                       int oi, int nb, int lane, f64x4 (&x)[CB]) {
+  const int li = lane & 15, kq = lane >> 4;
+#pragma unroll
+  for (int cb = 0; cb < CB; ++cb) x[cb] = f64x4{0, 0, 0, 0};
+  if (16 * oi >= nb) return;
+#pragma unroll
+  for (int jb = 0; jb < 4; ++jb) {
+    const bool use = (upper ? jb > oi : jb < oi) && 16 * jb < nb;  // wave-uniform
+    if (use) {
+      const double *sb = Is + sub_slot(upper, oi, jb);
+      double iq[4];
+#pragma unroll
+      for (int t4 = 0; t4 < 4; ++t4) iq[t4] = sb[li + 16 * (4 * t4 + kq)];
+#pragma unroll
+      for (int cb = 0; cb < CB; ++cb)
+#pragma unroll
+        for (int t4 = 0; t4 < 4; ++t4)
+          x[cb] = mma64(Bb[(16 * jb + 4 * t4 + kq) * kS + 16 * cb + li], iq[t4], x[cb]);
+    }
+  }
+  const double *sd = Is + sub_slot(upper, oi, oi);
+  double pd[4], qd[CB][4];
+  bool fin = true;
+#pragma unroll
+  for (int t4 = 0; t4 < 4; ++t4) {
+    const int j = 4 * t4 + kq;
+    pd[t4] = sd[li + 16 * j];
+#pragma unroll
+    for (int cb = 0; cb < CB; ++cb) {
+      qd[cb][t4] = Bb[(16 * oi + j) * kS + 16 * cb + li];
+      fin = fin && __builtin_isfinite(qd[cb][t4]);
+    }
+  }
+  if (__all(fin)) {
+#pragma unroll
+    for (int cb = 0; cb < CB; ++cb)
+#pragma unroll
+      for (int t4 = 0; t4 < 4; ++t4) x[cb] = mma64(qd[cb][t4], pd[t4], x[cb]);
+  } else {
+#pragma unroll
+    for (int cb = 0; cb < CB; ++cb)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int c = 16 * cb + kq + 4 * r;
+        double sum = x[cb][r];
+#pragma unroll
+        for (int jl = 0; jl < 16; ++jl) {
+          const int j = 16 * oi + jl;
+          const bool on = (upper ? jl >= li : jl <= li) && j < nb;
+          const double w = on ? sd[li + 16 * jl] : 0.0;
+          const double xv = on ? Bb[j * kS + c] : 0.0;
+          sum = on ? __builtin_fma(w, xv, sum) : sum;
+        }
+        x[cb][r] = sum;
+      }
+  }
+}
+
+// Diagonal block solved at step st.
+__device__ __forceinline__ int diag_block(bool forward, int nblk, int st) {
+  return forward ? st : nblk - 1 - st;
+}
+
+// Applies X_s to accumulator a (16 rows at row block start rb) from the
+// op(A) loads in flight in `cur` (CB = 1: the whole row block; CB = 2: its
+// first half), loading the next row block (rn, when `more`) into `nxt`.
+template <typename T, int CB>
+__device__ __forceinline__ void apply_x(const TrsmDesc<T> &d, f64x4 (&a)[CB],
+                                        T (&cur)[CB == 1 ? 16 : 8], T (&nxt)[CB == 1 ? 16 : 8],
+                                        const double (&xq)[16], const double *Xs, int kS, int rb,
+                                        int rn, bool more, int hi, int r0, int nb, int64_t si,
+                                        int64_t sj, int lane) {
+  if constexpr (CB == 1) {
+    if (more) load_rows_buf(d, rn, hi, r0, nb, si, sj, lane, nxt);
+    a[0] = update_acc(a[0], cur, xq);
+  } else {
+    load_half_buf(d, rb, hi, r0, nb, si, sj, 1, lane, nxt);
+    update_half(a, cur, Xs, kS, 0, lane);
+    if (more) load_half_buf(d, rn, hi, r0, nb, si, sj, 0, lane, cur);
+    update_half(a, nxt, Xs, kS, 1, lane);
+  }
+}
+
+// Units of 16 CB right-hand sides (CB = 1 or 2; orders up to 16 kWaves RB)
+// held in registers instead of LDS, as f64 MFMA accumulators. Row blocks (16
+// rows) are numbered h = 0, 1, ... in substitution order -- h = g for a
+// forward solve, nrb - 1 - g for a backward one (g the row block, nrb the
+// padded count) -- so that step s solves h in [4s, 4s + 4) (diagonal block
+// D_s) and updates h in [4s + 4, nrb). Row block h belongs to wave
+// h % kWaves; its j-th row block (h = wave + kWaves j) is held transposed in
+// CB accumulators: element r of lane l is B(16 g + (l & 15),
+// 16 cb + (l >> 4) + 4 r). An update is acc += (-X_s)^T op(A)[rows, block]^T
+// -- the X fragments are the product's first operand, the streamed op(A)
+// rows its second -- so B passes through LDS only for the 64 rows of a
+// diagonal block. The accumulators rotate: acc[0] is always the wave's first
+// row block not yet solved, and a solved one moves to the back, so every
+// index below is a constant.
+//
+// Look-ahead by one block: the four waves owning D_{s+1} apply X_s to those
+// rows first and stage them, so that X_{s+1} is formed while the other
+// waves are still applying X_s to the rest. Iteration s = -1 .. nblk - 2:
+//   A. every wave applies X_s to acc[0] (for the owners of D_{s+1}, their
+//      D_{s+1} rows), loading acc[1]'s op(A) rows; the owners stage acc[0]
+//      in Bb (rows at or past q as zeros);
+//   -- barrier (Bb staged; inv_{s+1} landed);
+//   C. inv_{s+2} starts for its Is buffer (LDS DMA); the owners form
+//      X_{s+1} = inv_{s+1} B_{s+1} (f64 MFMA; the scalar triangle for a
+//      diagonal sub-block with a non-finite B entry), keep it in acc[0] and
+//      publish -X_{s+1} in the other Xs buffer; every wave applies X_s to
+//      acc[1 ..], loading each row block's op(A) one block ahead (X
+//      fragments in registers for CB = 1, read from Xs per product for
+//      CB = 2), then starts the loads for its acc[0] of iteration s + 1;
+//   -- barrier (X_{s+1} visible; X_s, Bb and inv_s free);
+//   the owners rotate their solved acc[0] to the back.
+// Neither barrier waits for the op(A) loads in flight.
+template <typename T, int RB, int CB>
+__device__ __forceinline__ void solve_unit_reg(const TrsmDesc<T> &d, int c0, const double *inv,
+                                               double *Bs, double *Is0) {
+  static_assert(kWaves == 8, "owners of a diagonal block are waves 0-3 or 4-7");
+  constexpr int W = 16 * CB;
+  constexpr int kS = W + 1;                 // Bb / Xs row stride in doubles
+  constexpr int kLoads = CB == 1 ? 16 : 8;  // op(A) loads per row block (or half)
+  // LDS: Bb, Xs of even and odd steps, the second inverse buffer.
+  double *const Bb = Bs;
+  double *const Is1 = Bs + 3 * kNB * kS;
   // Opaque copy of the thread id: in the persistent loop every per-lane
   // offset is recomputed per unit instead of hoisted out of it (and spilled).
   int t = threadIdx.x;
@@ -626,189 +769,181 @@ __device__ void solve_unit_reg(const TrsmDesc<T> &d, int c0, const double *inv, 
   const int nc = min(W, d.nrhs - c0);
   const int qpad = (q + kNB - 1) / kNB * kNB;
   const int nrb = qpad / 16;
-  const bool upper = d.upper;
+  const bool upper = d.upper, fwd = d.forward;
   const int li = lane & 15, kq = lane >> 4;
   const int nblk = qpad / kNB;
-  // The first diagonal block's inverse (lands before the first barrier).
-  dma_inverse(inv + d.inv0 + (int64_t)(d.forward ? 0 : nblk - 1) * kNB * kNB, upper, Is, wave,
+  // Accumulators of this wave with h >= 4 st: h = wave + kWaves j, j < nrb.
+  auto live_from = [](int st, int w, int n) {
+    return n > w ? (n - w + kWaves - 1) / kWaves - (4 * st > w ? (4 * st - w + kWaves - 1) / kWaves : 0)
+                 : 0;
+  };
+  const int nown = live_from(0, wave, nrb);  // this wave's row blocks
+  dma_inverse(inv + d.inv0 + (int64_t)diag_block(fwd, nblk, 0) * kNB * kNB, upper, Is0, wave,
               lane);
   // B, alpha applied to every entry on load (rblas's pre-scale); row blocks
-  // past the order are neither loaded nor stored. Lane addresses advance by
-  // wave-uniform steps: 16 kWaves rows per accumulator, 4 columns per r.
+  // past the order are neither loaded nor stored.
   const int64_t rstep = d.right ? d.ldb : 1, cstep = d.right ? 1 : d.ldb;
-  const T *bl = d.b + (int64_t)(16 * wave + li) * rstep + (int64_t)(c0 + kq) * cstep;
-  f64x4 acc[RB];
+  T *const bl = d.b + (int64_t)li * rstep + (int64_t)(c0 + kq) * cstep;
+  f64x4 acc[RB][CB];
   {
-    T raw[RB][4];
+    T raw[RB][CB][4];
 #pragma unroll
     for (int j = 0; j < RB; ++j) {
-      const bool rok = 16 * (wave + kWaves * j) + li < q;
-      if (16 * (wave + kWaves * j) < q) {
+      const int g = row_block(upper, nrb, wave, j);
+      const bool live = j < nown && 16 * g < q;
+      const bool rok = 16 * g + li < q;
 #pragma unroll
-        for (int r = 0; r < 4; ++r)
-          if (rok && kq + 4 * r < nc)
-            raw[j][r] = bl[(int64_t)(16 * kWaves * j) * rstep + (int64_t)(4 * r) * cstep];
-      }
+      for (int cb = 0; cb < CB; ++cb)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          raw[j][cb][r] = T(0);
+          if (live && rok && 16 * cb + kq + 4 * r < nc)
+            raw[j][cb][r] = bl[(int64_t)(16 * g) * rstep + (int64_t)(16 * cb + 4 * r) * cstep];
+        }
     }
 #pragma unroll
     for (int j = 0; j < RB; ++j) {
-      const bool rok = 16 * (wave + kWaves * j) + li < q;
-      if (16 * (wave + kWaves * j) < q) {
+      const int g = row_block(upper, nrb, wave, j);
+      const bool rok = j < nown && 16 * g + li < q;
+#pragma unroll
+      for (int cb = 0; cb < CB; ++cb)
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-          const T v = d.alpha != T(1) ? d.alpha * raw[j][r] : raw[j][r];
-          acc[j][r] = (rok && kq + 4 * r < nc) ? (double)v : 0.0;
+          const T v = d.alpha != T(1) ? d.alpha * raw[j][cb][r] : raw[j][cb][r];
+          acc[j][cb][r] = (rok && 16 * cb + kq + 4 * r < nc) ? (double)v : 0.0;
         }
-      } else {
-        acc[j] = f64x4{0, 0, 0, 0};
-      }
     }
   }
   const int64_t si = d.trans ? d.lda : 1;  // op(A) row step
   const int64_t sj = d.trans ? 1 : d.lda;  // op(A) column step
-  for (int s = 0; s < nblk; ++s) {
-    const int b = d.forward ? s : nblk - 1 - s;
-    const int r0 = b * kNB;
-    const int nb = min(kNB, q - r0);
+  T pa[kLoads], pb[kLoads];
+  double xq[16];
+  int done = 0;  // rotations so far: acc[k] holds the wave's row block j = k + done
+  for (int s = -1; s + 1 < nblk; ++s) {  // the last block's step has nothing left to update
+    const bool upd = s >= 0;
+    const int r0 = upd ? diag_block(fwd, nblk, s) * kNB : 0;
+    const int nb = upd ? min(kNB, q - r0) : 0;
     const int hi = upper ? r0 : q;
-    // Row blocks of the update: [4b + 4, nrb) (lower) or [0, 4b) (upper);
-    // this wave's accumulators among them: j in [ja, jz).
-    const int glo = upper ? 0 : 4 * b + 4, ghi = upper ? 4 * b : nrb;
-    const int ja = glo > wave ? (glo - wave + kWaves - 1) / kWaves : 0;
-    const int jz = ghi > wave ? (ghi - wave + kWaves - 1) / kWaves : 0;
-    // Owners: row block 4b + oi (oi < 4) is wave (4b) % kWaves + oi's
-    // accumulator oj.
-    const int oi = wave - ((4 * b) % kWaves), oj = (4 * b) / kWaves;
-    const bool owner = oi >= 0 && oi < 4;
-    // 1.
-    if (owner) {
-      f64x4 v = acc[0];
+    const int nact = upd ? live_from(s + 1, wave, nrb) : 0;  // accumulators X_s reaches
+    // Owners of D_{s+1}: h = 4s + 4 + i4, this wave's acc[0].
+    const bool next = s + 1 < nblk;
+    const int i4 = wave - ((4 * s + 4) % kWaves);
+    const bool owner = next && i4 >= 0 && i4 < 4;
+    const int oi = upper ? 3 - i4 : i4;  // sub-block within D_{s+1}
+    const int r0n = next ? diag_block(fwd, nblk, s + 1) * kNB : 0;
+    const int nbn = next ? min(kNB, q - r0n) : 0;
+    const double *Xs = Bs + (1 + (s & 1)) * kNB * kS;
+    if (CB == 1 && upd) {
 #pragma unroll
-      for (int j = 1; j < RB; ++j)
-        if (j == oj) v = acc[j];
+      for (int t4 = 0; t4 < 16; ++t4) xq[t4] = Xs[(4 * t4 + kq) * kS + li];
+    }
+    // A.
+    bool more = false;
+    if (nact > 0) {
+      more = nact > 1;
+      apply_x<T, CB>(d, acc[0], pa, pb, xq, Xs, kS, 16 * row_block(upper, nrb, wave, done),
+                     16 * row_block(upper, nrb, wave, done + 1), more, hi, r0, nb, si, sj, lane);
+    }
+    if (owner) {
       const int row = 16 * oi + li;
 #pragma unroll
-      for (int r = 0; r < 4; ++r) Bb[row * kS + kq + 4 * r] = r0 + row < q ? v[r] : 0.0;
+      for (int cb = 0; cb < CB; ++cb)
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+          Bb[row * kS + 16 * cb + kq + 4 * r] = r0n + row < q ? acc[0][cb][r] : 0.0;
     }
-    T pa[16], pb[16];
-    if (ja < jz) {
-      const int rb = 16 * (wave + kWaves * ja);
-      if (ja & 1)
-        load_rows_buf(d, rb, hi, r0, nb, si, sj, lane, pb);
-      else
-        load_rows_buf(d, rb, hi, r0, nb, si, sj, lane, pa);
-      barrier_vm<16>();  // B_b staged, inv_b landed; the row block stays in flight
-    } else {
+    if (more)
+      barrier_vm<kLoads>();  // B_{s+1} staged; inv_{s+1} landed
+    else
       barrier_vm<0>();
-    }
-    // 2.
+    // C.
+    if (s + 2 < nblk)
+      dma_inverse(inv + d.inv0 + (int64_t)diag_block(fwd, nblk, s + 2) * kNB * kNB, upper,
+                  (s & 1) ? Is1 : Is0, wave, lane);
     if (owner) {
-      f64x4 x = {0, 0, 0, 0};
-      if (16 * oi < nb) {
-#pragma unroll
-        for (int jb = 0; jb < 4; ++jb) {
-          const bool use = (upper ? jb > oi : jb < oi) && 16 * jb < nb;  // wave-uniform
-          if (use) {
-            const double *sb = Is + sub_slot(upper, oi, jb);
-            double iq[4], bp[4];
-#pragma unroll
-            for (int t4 = 0; t4 < 4; ++t4) {
-              const int j = 4 * t4 + kq;
-              iq[t4] = sb[li + 16 * j];
-              bp[t4] = Bb[(16 * jb + j) * kS + li];
-            }
-#pragma unroll
-            for (int t4 = 0; t4 < 4; ++t4) x = mma64(bp[t4], iq[t4], x);
-          }
-        }
-        // Diagonal sub-block: the full 16 x 16 product when its B rows are
-        // all finite (the inverse's exact zeros outside the triangle add
-        // exact zeros), else the triangle alone, scalar f64, so a non-finite
-        // entry reaches exactly the X entries rblas's substitution reaches.
-        const double *sd = Is + sub_slot(upper, oi, oi);
-        double pd[4], qd[4];
-        bool fin = true;
-#pragma unroll
-        for (int t4 = 0; t4 < 4; ++t4) {
-          const int j = 4 * t4 + kq;
-          pd[t4] = sd[li + 16 * j];
-          qd[t4] = Bb[(16 * oi + j) * kS + li];
-          fin = fin && __builtin_isfinite(qd[t4]);
-        }
-        if (__all(fin)) {
-#pragma unroll
-          for (int t4 = 0; t4 < 4; ++t4) x = mma64(qd[t4], pd[t4], x);
-        } else {
-#pragma unroll
-          for (int r = 0; r < 4; ++r) {
-            const int c = kq + 4 * r;
-            double sum = x[r];
-#pragma unroll
-            for (int jl = 0; jl < 16; ++jl) {
-              const int j = 16 * oi + jl;
-              const bool on = (upper ? jl >= li : jl <= li) && j < nb;
-              const double w = on ? sd[li + 16 * jl] : 0.0;
-              const double xv = on ? Bb[j * kS + c] : 0.0;
-              sum = on ? __builtin_fma(w, xv, sum) : sum;
-            }
-            x[r] = sum;
-          }
-        }
-      }
-#pragma unroll
-      for (int j = 0; j < RB; ++j)
-        if (j == oj) acc[j] = x;
+      f64x4 x[CB];
+      form_x<CB>(Bb, kS, (s & 1) ? Is0 : Is1, upper, oi, nbn, lane, x);
+      double *Xn = Bs + (1 + ((s + 1) & 1)) * kNB * kS;
       const int row = 16 * oi + li;
 #pragma unroll
-      for (int r = 0; r < 4; ++r) Xs[row * kS + kq + 4 * r] = -x[r];
-    }
-    barrier_vm<16>();  // -X_b visible; every read of Is and Bb done
-    // 4. inv_{b+1} straight into Is, landing before the next step's first
-    // barrier.
-    if (s + 1 < nblk)
-      dma_inverse(inv + d.inv0 + (int64_t)(d.forward ? b + 1 : b - 1) * kNB * kNB, upper, Is,
-                  wave, lane);
-    // 3.
-    double xq[16];
+      for (int cb = 0; cb < CB; ++cb) {
+        acc[0][cb] = x[cb];
 #pragma unroll
-    for (int t4 = 0; t4 < 16; ++t4) {
-      const int j = 4 * t4 + kq;
-      xq[t4] = j < nb ? Xs[j * kS + li] : 0.0;
-    }
-#pragma unroll
-    for (int j = 0; j < RB; ++j) {
-      if (j >= ja && j < jz) {
-        const int rb = 16 * (wave + kWaves * j);
-        if (j & 1) {
-          if (j + 1 < jz) load_rows_buf(d, rb + 16 * kWaves, hi, r0, nb, si, sj, lane, pa);
-          acc[j] = update_acc(acc[j], pb, xq);
-        } else {
-          if (j + 1 < jz) load_rows_buf(d, rb + 16 * kWaves, hi, r0, nb, si, sj, lane, pb);
-          acc[j] = update_acc(acc[j], pa, xq);
-        }
+        for (int r = 0; r < 4; ++r)
+          Xn[row * kS + 16 * cb + kq + 4 * r] = row < nbn ? -x[cb][r] : 0.0;
       }
+    }
+#pragma unroll
+    for (int k = 1; k < RB; ++k) {
+      if (k < nact) {
+        const int rb = 16 * row_block(upper, nrb, wave, done + k);
+        const int rn = 16 * row_block(upper, nrb, wave, done + k + 1);
+        // CB = 1 alternates the two sets; CB = 2 always starts from pa.
+        if (CB == 1 && (k & 1))
+          apply_x<T, CB>(d, acc[k], pb, pa, xq, Xs, kS, rb, rn, k + 1 < nact, hi, r0, nb, si, sj,
+                         lane);
+        else
+          apply_x<T, CB>(d, acc[k], pa, pb, xq, Xs, kS, rb, rn, k + 1 < nact, hi, r0, nb, si, sj,
+                         lane);
+      }
+    }
+    // Loads for acc[0] of iteration s + 1 (after this iteration's rotation).
+    const int first = done + (owner ? 1 : 0);
+    bool inflight = false;
+    if (s + 1 < nblk && live_from(s + 2, wave, nrb) > 0) {
+      const int r0x = diag_block(fwd, nblk, s + 1) * kNB;
+      if constexpr (CB == 1)
+        load_rows_buf(d, 16 * row_block(upper, nrb, wave, first), upper ? r0x : q, r0x,
+                      min(kNB, q - r0x), si, sj, lane, pa);
+      else
+        load_half_buf(d, 16 * row_block(upper, nrb, wave, first), upper ? r0x : q, r0x,
+                      min(kNB, q - r0x), si, sj, 0, lane, pa);
+      inflight = true;
+    }
+    if (inflight)
+      barrier_vm<kLoads>();  // X_{s+1} visible
+    else
+      barrier_vm<0>();
+    if (owner) {  // acc[0] solved: rotate it to the back
+      f64x4 tmp[CB];
+#pragma unroll
+      for (int cb = 0; cb < CB; ++cb) tmp[cb] = acc[0][cb];
+#pragma unroll
+      for (int k = 0; k + 1 < RB; ++k)
+#pragma unroll
+        for (int cb = 0; cb < CB; ++cb) acc[k][cb] = acc[k + 1][cb];
+#pragma unroll
+      for (int cb = 0; cb < CB; ++cb) acc[RB - 1][cb] = tmp[cb];
+      ++done;
     }
   }
-  T *bo = d.b + (int64_t)(16 * wave + li) * rstep + (int64_t)(c0 + kq) * cstep;
+  // After `done` rotations acc[k] holds the wave's row block j = (k + done) % RB.
 #pragma unroll
-  for (int j = 0; j < RB; ++j) {
-    const bool rok = 16 * (wave + kWaves * j) + li < q;
-    if (16 * (wave + kWaves * j) < q) {
+  for (int k = 0; k < RB; ++k) {
+    const int j = (k + done) % RB;
+    const int g = row_block(upper, nrb, wave, j);
+    const bool rok = j < nown && 16 * g + li < q;
+    if (j < nown && 16 * g < q) {
 #pragma unroll
-      for (int r = 0; r < 4; ++r)
-        if (rok && kq + 4 * r < nc)
-          bo[(int64_t)(16 * kWaves * j) * rstep + (int64_t)(4 * r) * cstep] = (T)acc[j][r];
+      for (int cb = 0; cb < CB; ++cb)
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+          if (rok && 16 * cb + kq + 4 * r < nc)
+            bl[(int64_t)(16 * g) * rstep + (int64_t)(16 * cb + 4 * r) * cstep] = (T)acc[k][cb][r];
     }
   }
 }
 
-// Width 16 through the register-resident solve, width 8 through the LDS one
-// (trsm_unit_width picks).
+// Widths 32 and 16 through the register-resident solve, width 8 through the
+// LDS one (trsm_unit_width picks).
 template <typename T>
 __device__ __forceinline__ void solve_any(const TrsmDesc<T> &d, int c0, const double *inv,
                                           double *Bs, double *Is) {
-  if (d.width == 16)
-    solve_unit_reg<T, kTrsmLdsDoubles / 16 / (16 * kWaves)>(d, c0, inv, Bs, Bs + kNB * 17, Is);
+  constexpr int RB = kTrsmLdsDoubles / 16 / (16 * kWaves);
+  if (d.width == 32)
+    solve_unit_reg<T, kTrsmWideMaxOrder / (16 * kWaves), 2>(d, c0, inv, Bs, Is);
+  else if (d.width == 16)
+    solve_unit_reg<T, RB, 1>(d, c0, inv, Bs, Is);
   else
     solve_unit<T, 8>(d, c0, inv, Bs, Is);
 }
@@ -822,16 +957,17 @@ template <typename T>
 __global__ __launch_bounds__(kThreads) void grouped_trsm_kernel(const TrsmDesc<T> *descs,
                                                                 const int2 *units, int *sched,
                                                                 const double *inv) {
-  __shared__ double Bs[kLdsDoubles];
+  __shared__ __align__(16) double Bs[kLdsDoubles];
   __shared__ __align__(16) double Is[kInvLds];
   __shared__ int s_unit;
   const int home = (int)(blockIdx.x % kTrsmQueues);
-  // Thread 0 keeps the queue offsets and whether its own queue has run dry.
-  int qs[kTrsmQueues + 1];
-  bool home_open = true;
+  // Queue offsets and whether this workgroup's own queue has run dry, in LDS
+  // (read by thread 0 only; kept out of the solve's registers).
+  __shared__ int qs[kTrsmQueues + 1];
+  __shared__ int home_open;
   if (threadIdx.x == 0) {
-#pragma unroll
     for (int x = 0; x <= kTrsmQueues; ++x) qs[x] = sched[kTrsmQueueStart + x];
+    home_open = 1;
   }
   for (;;) {
     if (threadIdx.x == 0) {
@@ -841,7 +977,7 @@ __global__ __launch_bounds__(kThreads) void grouped_trsm_kernel(const TrsmDesc<T
         if (c < qs[home + 1] - qs[home])
           u = qs[home] + c;
         else
-          home_open = false;
+          home_open = 0;
       }
       // Own queue drained: read every counter at once and help the queue
       // furthest behind, if it still holds kStealMin units (fewer are left to
@@ -886,7 +1022,7 @@ __global__ __launch_bounds__(kThreads) void grouped_trsm_kernel(const TrsmDesc<T
 // u*width .. u*width + width - 1.
 template <typename T>
 __global__ __launch_bounds__(kThreads) void single_trsm_solve(TrsmDesc<T> d, const double *inv) {
-  __shared__ double Bs[kLdsDoubles];
+  __shared__ __align__(16) double Bs[kLdsDoubles];
   __shared__ __align__(16) double Is[kInvLds];
   solve_any(d, (int)blockIdx.x * d.width, inv, Bs, Is);
 }
