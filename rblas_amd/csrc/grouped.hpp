@@ -75,23 +75,23 @@ constexpr int kTrsmQueueStart = (kTrsmQueues + 1) * kTrsmLine;
 constexpr int kTrsmSchedBytes = 1280;
 static_assert((kTrsmQueueStart + kTrsmQueues + 1) * 4 <= kTrsmSchedBytes, "sched block");
 
-// Right-hand sides per trsm work unit. Held in registers: 32 for orders up
-// to kTrsmWideMaxOrder (each op(A) row block then feeds twice the MFMAs),
-// else 16 for orders up to 1024 -- both need 32-bit buffer offsets up to
-// (63 lda + 15) elements for their op(A) row loads. Held in LDS: 8, orders up
-// to 2048. Width 32 is taken when a system has more than 16 right-hand sides
-// and, with fill_first (one system alone on the GPU), only when width-16
-// units would outnumber the CUs.
-#ifndef RBLAS_TRSM_WIDE_MAX
-#define RBLAS_TRSM_WIDE_MAX 512
-#endif
-constexpr int kTrsmWideMaxOrder = RBLAS_TRSM_WIDE_MAX;
+// Right-hand sides per trsm work unit. Held in registers, orders up to 1024:
+// 16, or 32 (each op(A) row block then feeds twice the MFMAs; fewer units);
+// both need 32-bit buffer offsets up to (63 lda + 15) elements for their
+// op(A) row loads. Held in LDS: 8, orders up to 2048. trsm_unit_width gives
+// 16 or 8; trsm_wide_ok says whether 32 is possible, which a caller takes
+// only while the units still fill about three quarters of the CUs
+// (kTrsmFillUnits).
+constexpr int kTrsmWideMaxOrder = 1024;
+constexpr int kTrsmFillUnits = 3 * kCUs / 4;
 template <typename T>
-inline int trsm_unit_width(int64_t q, int64_t lda, int64_t nrhs, bool fill_first) {
+inline int trsm_unit_width(int64_t q, int64_t lda) {
   const bool fits = lda * 63 + 15 < ((int64_t)1 << 31) / (int64_t)sizeof(T);
-  if (q > kTrsmLdsDoubles / 16 || !fits) return 8;
-  const bool wide = q <= kTrsmWideMaxOrder && nrhs > 16 && (!fill_first || nrhs > 16 * kCUs);
-  return wide ? 32 : 16;
+  return q <= kTrsmLdsDoubles / 16 && fits ? 16 : 8;
+}
+template <typename T>
+inline bool trsm_wide_ok(int64_t q, int64_t lda, int64_t nrhs) {
+  return trsm_unit_width<T>(q, lda) == 16 && q <= kTrsmWideMaxOrder && nrhs > 16;
 }
 
 template <typename T>

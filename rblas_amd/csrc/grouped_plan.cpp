@@ -187,28 +187,52 @@ void TrsmGroup<T>::add(bool left, bool trans, bool eff_upper, bool unit, int64_t
   d.unit = unit;
   d.right = !left;
   d.forward = !d.upper;
-  d.width = trsm_unit_width<T>(d.q, d.lda, d.nrhs, false);
+  d.width = trsm_unit_width<T>(d.q, d.lda);  // finish() may widen it to 32
   d.alpha = alpha;
   d.inv0 = inv_elems;
   const int prob = (int)descs.size();
   const int nblk = (d.q + 63) / 64;
   inv_elems += (int64_t)nblk * 64 * 64;
   for (int blk = 0; blk < nblk; ++blk) blocks.push_back(int2{prob, blk});
-  // Unit time model (us of one CU, f64, measured on 16-wide units): a fixed
-  // 3 us, 3.5 us per 64-row diagonal block (its barrier-separated steps), and
-  // the update at about 0.2 TFLOP/s per CU.
-  const double unit_us = 3.0 + 3.5 * nblk + (double)d.q * (double)d.q * d.width / 2.0e5;
-  for (int c0 = 0; c0 < d.nrhs; c0 += d.width) {
-    units.push_back(int2{prob, c0});
-    cost.push_back(unit_us);
-  }
   info.flops += (double)d.q * (double)d.q * (double)d.nrhs;
   descs.push_back(d);
 }
 
 template <typename T>
 void TrsmGroup<T>::finish() {
-  // add() appends each problem's units consecutively, all of one cost.
+  // Widths: 32 for the largest eligible systems first, while the units still
+  // fill kTrsmFillUnits CUs.
+  int64_t total = 0;
+  std::vector<int> wide;
+  for (size_t p = 0; p < descs.size(); ++p) {
+    const TrsmDesc<T> &d = descs[p];
+    total += (d.nrhs + d.width - 1) / d.width;
+    if (trsm_wide_ok<T>(d.q, d.lda, d.nrhs)) wide.push_back((int)p);
+  }
+  std::stable_sort(wide.begin(), wide.end(),
+                   [&](int x, int y) { return descs[x].q > descs[y].q; });
+  for (int p : wide) {
+    TrsmDesc<T> &d = descs[p];
+    const int64_t after = total - (d.nrhs + 15) / 16 + (d.nrhs + 31) / 32;
+    if (after < kTrsmFillUnits) continue;
+    d.width = 32;
+    total = after;
+  }
+  // Units, each problem's consecutively. Time model (us of one CU, f64): a
+  // fixed 3 us, 3.5 us per 64-row diagonal block (its barrier-separated
+  // steps), and the update at about 0.2 TFLOP/s per CU.
+  units.clear();
+  cost.clear();
+  for (size_t p = 0; p < descs.size(); ++p) {
+    const TrsmDesc<T> &d = descs[p];
+    const int nblk = (d.q + 63) / 64;
+    const double unit_us = 3.0 + 3.5 * nblk + (double)d.q * (double)d.q * d.width / 2.0e5;
+    for (int c0 = 0; c0 < d.nrhs; c0 += d.width) {
+      units.push_back(int2{(int)p, c0});
+      cost.push_back(unit_us);
+    }
+  }
+  // Each problem's units are consecutive, all of one cost.
   struct Chunk {
     size_t u0, n;
     double unit_cost;

@@ -518,7 +518,8 @@ hipError_t launch_trsm(const TrsmProblem<T> &p, void *ws, hipStream_t st) {
     d.unit = p.unit;
     d.right = !p.left;
     d.forward = !d.upper;
-    d.width = trsm_unit_width<T>(q, p.lda, d.nrhs, true);
+    d.width = trsm_unit_width<T>(q, p.lda);
+    if (trsm_wide_ok<T>(q, p.lda, d.nrhs) && (d.nrhs + 31) / 32 >= kTrsmFillUnits) d.width = 32;
     d.alpha = p.alpha;
     d.inv0 = 0;
     return launch_trsm_single<T>(d, (double *)ws, st);
