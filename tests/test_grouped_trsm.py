@@ -180,6 +180,69 @@ def test_grouped_trsm_matches_single_calls_and_replays(gpu):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_grouped_trsm_wide_units_match_single_calls(gpu, dtype):
+    """48 systems of order up to 1024 with 100..250 right-hand sides: the
+    group runs them as 32-wide units (its planner widens them while the units
+    still fill the CUs), a single call as 16-wide ones. Each right-hand side
+    is computed by the same sequence of f64 operations in both, so the
+    results agree bit for bit; they also meet the residual bound."""
+    rng = np.random.default_rng(31)
+    s = Stream()
+    probs = []
+    combos = list(itertools.product((0, 1), (0, 1), (0, 1, 2), (0, 1)))
+    for i in range(48):
+        side, uplo, trans, diag = combos[i % len(combos)]
+        q = int(np.exp(rng.uniform(np.log(20), np.log(1024))))
+        nr = int(rng.integers(100, 251))
+        m, n = (q, nr) if side == 0 else (nr, q)
+        lda, ldb = q + 1, m + 2
+        a, opA = _system(q, uplo, trans, diag, lda, dtype, rng)
+        b = rng.standard_normal((n - 1) * ldb + m).astype(dtype)
+        probs.append(dict(side=side, uplo=uplo, trans=trans, diag=diag, m=m, n=n, alpha=0.75,
+                          a=a, lda=lda, b=b, ldb=ldb, opA=opA, q=q))
+    dev = _upload(probs, s)
+    plan = grouped.GroupedTrsm(dev, dtype)
+    wide_units = sum(((p["n"] if p["side"] == 0 else p["m"]) + 31) // 32 for p in probs)
+    assert plan.info()["units"] == wide_units
+    plan.run(s)
+    s.synchronize()
+    for i, (p, d) in enumerate(zip(probs, dev)):
+        got = d.b.download(s)
+        b1 = p["b"].copy()
+        level3.trsm(p["side"], p["uplo"], p["trans"], p["diag"], p["m"], p["n"], p["alpha"],
+                    p["a"], p["lda"], b1, p["ldb"])
+        assert np.array_equal(got, b1), (i, p["side"], p["uplo"], p["trans"], p["q"])
+        if i % 6 == 0:
+            X = view(got, COL, p["m"], p["n"], p["ldb"]).astype(np.float64)
+            B0 = view(p["b"], COL, p["m"], p["n"], p["ldb"]).astype(np.float64)
+            assert _residual_ratio(p["side"], p["opA"], X, p["alpha"], B0, p["q"], dtype) <= 1.0
+    plan.close()
+    s.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("side,uplo,trans", list(itertools.product((0, 1), (0, 1), (0, 1))))
+def test_trsm_single_call_wide_units_exact(gpu, side, uplo, trans):
+    """A single call with 6200 right-hand sides takes 32-wide units (16-wide
+    ones would be more than the CUs need): bidiagonal op(A) with entries in
+    {-1, 0, 1}, diagonal +-1, order 300, exact integer solution bit for bit."""
+    q, nr = 300, 6200
+    m, n = (q, nr) if side == 0 else (nr, q)
+    rng = np.random.default_rng(40 + 4 * side + 2 * uplo + trans)
+    S = np.diag(rng.choice([-1.0, 1.0], q))
+    off = rng.integers(-1, 2, q - 1).astype(np.float64)
+    eff_upper = (uplo == 0) != bool(trans)
+    opA = S + (np.diag(off, 1) if eff_upper else np.diag(off, -1))
+    stored = opA.T if trans else opA
+    X = rng.integers(-4, 5, (m, n)).astype(np.float64)
+    B = opA @ X if side == 0 else X @ opA
+    b = B.T.reshape(-1).copy()
+    level3.trsm(side, uplo, trans, 0, m, n, 1.0, stored.T.reshape(-1).copy(), q, b, m)
+    assert np.array_equal(b.reshape(n, m).T, X)
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("side,uplo,trans", list(itertools.product((0, 1), (0, 1), (0, 1))))
 def test_trsm_nan_reaches_what_rblas_reaches(gpu, oracle, side, uplo, trans):
     """One NaN in B: the set of NaN results equals the oracle's (rblas's
