@@ -198,6 +198,14 @@ void TrsmGroup<T>::add(bool left, bool trans, bool eff_upper, bool unit, int64_t
   descs.push_back(d);
 }
 
+// Time model of one trsm unit, us of one CU (f64), fitted to the unit times
+// of the batching workload with every CU busy: 4.8 us fixed, 4.8 us per
+// 64-row diagonal block (its barrier-separated steps), and the update at
+// about 0.19 TFLOP/s per CU.
+static double trsm_unit_us(int q, int width) {
+  return 4.8 + 4.8 * ((q + 63) / 64) + (double)q * (double)q * width / 1.87e5;
+}
+
 template <typename T>
 void TrsmGroup<T>::finish() {
   // Widths: 32 for the largest eligible systems first, while the units still
@@ -213,20 +221,19 @@ void TrsmGroup<T>::finish() {
                    [&](int x, int y) { return descs[x].q > descs[y].q; });
   for (int p : wide) {
     TrsmDesc<T> &d = descs[p];
-    const int64_t after = total - (d.nrhs + 15) / 16 + (d.nrhs + 31) / 32;
+    const int64_t n16 = (d.nrhs + 15) / 16, n32 = (d.nrhs + 31) / 32;
+    const int64_t after = total - n16 + n32;
     if (after < kTrsmFillUnits) continue;
     d.width = 32;
     total = after;
   }
-  // Units, each problem's consecutively. Time model (us of one CU, f64): a
-  // fixed 3 us, 3.5 us per 64-row diagonal block (its barrier-separated
-  // steps), and the update at about 0.2 TFLOP/s per CU.
+
+  // Units, each problem's consecutively.
   units.clear();
   cost.clear();
   for (size_t p = 0; p < descs.size(); ++p) {
     const TrsmDesc<T> &d = descs[p];
-    const int nblk = (d.q + 63) / 64;
-    const double unit_us = 3.0 + 3.5 * nblk + (double)d.q * (double)d.q * d.width / 2.0e5;
+    const double unit_us = trsm_unit_us(d.q, d.width);
     for (int c0 = 0; c0 < d.nrhs; c0 += d.width) {
       units.push_back(int2{(int)p, c0});
       cost.push_back(unit_us);

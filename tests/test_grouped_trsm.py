@@ -184,9 +184,9 @@ def test_grouped_trsm_matches_single_calls_and_replays(gpu):
 def test_grouped_trsm_wide_units_match_single_calls(gpu, dtype):
     """48 systems of order up to 1024 with 100..250 right-hand sides: the
     group runs them as 32-wide units (its planner widens them while the units
-    still fill the CUs), a single call as 16-wide ones. Each right-hand side
-    is computed by the same sequence of f64 operations in both, so the
-    results agree bit for bit; they also meet the residual bound."""
+    still fill the CUs), a single call as 16-wide ones. Each right-hand side is computed by the same
+    sequence of f64 operations at either width, so the results agree bit for
+    bit; they also meet the residual bound."""
     rng = np.random.default_rng(31)
     s = Stream()
     probs = []
@@ -203,8 +203,9 @@ def test_grouped_trsm_wide_units_match_single_calls(gpu, dtype):
                           a=a, lda=lda, b=b, ldb=ldb, opA=opA, q=q))
     dev = _upload(probs, s)
     plan = grouped.GroupedTrsm(dev, dtype)
-    wide_units = sum(((p["n"] if p["side"] == 0 else p["m"]) + 31) // 32 for p in probs)
-    assert plan.info()["units"] == wide_units
+    nrhs = [p["n"] if p["side"] == 0 else p["m"] for p in probs]
+    units16 = sum((r + 15) // 16 for r in nrhs)
+    assert sum((r + 31) // 32 for r in nrhs) <= plan.info()["units"] < units16  # some widened
     plan.run(s)
     s.synchronize()
     for i, (p, d) in enumerate(zip(probs, dev)):
