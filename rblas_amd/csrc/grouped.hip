@@ -17,73 +17,56 @@
 namespace rblas {
 namespace {
 
-// Descriptor fields are loaded with vector loads, so the compiler cannot see
-// that they are the same in every lane; readfirstlane makes them scalar again,
-// so buffer resources and LDS addresses built from them need no per-lane loop.
-__device__ __forceinline__ int uni(int x) { return __builtin_amdgcn_readfirstlane(x); }
-__device__ __forceinline__ int64_t uni(int64_t x) {
-  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)(uint64_t)x);
-  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)((uint64_t)x >> 32));
-  return (int64_t)(((uint64_t)hi << 32) | lo);
-}
-template <typename P>
-__device__ __forceinline__ P *uni(P *p) {
-  return (P *)uni((int64_t)p);
-}
-
-// The descriptor, loaded once and made wave-uniform field by field.
-template <typename T>
-__device__ __forceinline__ GroupDesc<T> load_desc(const GroupDesc<T> *dp) {
-  GroupDesc<T> d = *dp;
-  d.a = uni(d.a);
-  d.b = uni(d.b);
-  d.c = uni(d.c);
-  d.lda = uni(d.lda);
-  d.ldb = uni(d.ldb);
-  d.ldc = uni(d.ldc);
-  d.m = uni(d.m);
-  d.n = uni(d.n);
-  d.k = uni(d.k);
-  d.splits = uni(d.splits);
-  d.tiles_m = uni(d.tiles_m);
-  d.slot0 = uni(d.slot0);
-  d.cnt0 = uni(d.cnt0);
-  return d;
+// Descriptors and work units are never written by the launch, so they are
+// read through the constant address space: uniform addresses then load with
+// scalar loads straight into SGPRs.
+template <typename X>
+__device__ __forceinline__ X const_load(const X *p) {
+  static_assert(sizeof(X) % 4 == 0, "whole words");
+  typedef const __attribute__((address_space(4))) uint32_t cword;
+  const cword *q = (const cword *)p;
+  X x;
+  uint32_t *w = (uint32_t *)&x;
+#pragma unroll
+  for (int i = 0; i < (int)(sizeof(X) / 4); ++i) w[i] = q[i];
+  return x;
 }
 
 // One unit of one variant (transposes, tile side). TileGemm::init derives
 // its lane addresses from an opaque thread id, so they are recomputed per
 // unit rather than hoisted out of the persistent loop for all eight variants
-// at once (which would need more registers than a wave has).
+// at once (which would need more registers than a wave has). Once the k loop
+// is done (its last barrier drained every load, the claim's atomic with
+// them), thread 0 publishes the next unit's index in *s_next.
 template <typename T, bool TA, bool TB, int BM>
-__device__ __forceinline__ void run_product(const GroupDesc<T> *dp, int tile, int split, T *ws,
-                                            int *counters) {
+__device__ __forceinline__ void run_product(const GroupDesc<T> &d, const WorkUnit &w, T *ws,
+                                            int *counters, int next, int *s_next) {
   typedef Geo<T> G;
-  const GroupDesc<T> d = load_desc(dp);
   GemmProblem<T> p{TA, TB, d.m, d.n, d.k, d.alpha, d.beta, d.a, d.lda, d.b, d.ldb, d.c, d.ldc};
-  const int tm = tile % d.tiles_m, tn = tile / d.tiles_m;
+  const int tm = w.tile % d.tiles_m, tn = w.tile / d.tiles_m;
   const int m0 = tm * BM, n0 = tn * BM;
   const int ktiles = (d.k + G::BK - 1) / G::BK;
   const int per = (ktiles + d.splits - 1) / d.splits;
-  const int kb = min(d.k, split * per * G::BK), ke = min(d.k, (split + 1) * per * G::BK);
+  const int kb = min(d.k, w.split * per * G::BK), ke = min(d.k, (w.split + 1) * per * G::BK);
   TileGemm<T, TA, TB, BM, BM> tg;
   tg.init(p, m0, n0, kb, ke);
   tg.mainloop();
+  if (threadIdx.x == 0) *s_next = next;
   if (d.splits > 1 &&
-      !split_handoff(tg, ws + d.slot0, (int64_t)tile * d.splits, d.splits, split,
-                     counters + d.cnt0 + tile))
+      !split_handoff(tg, ws + d.slot0, (int64_t)w.tile * d.splits, d.splits, w.split,
+                     counters + d.cnt0 + w.tile))
     return;
   tg.epilogue(p, m0, n0);
 }
 
 template <typename T, int BM>
-__device__ __forceinline__ void run_tile(const GroupDesc<T> *dp, int ta, int tb, int tile,
-                                         int split, T *ws, int *counters) {
-  switch (ta * 2 + tb) {
-    case 0: run_product<T, false, false, BM>(dp, tile, split, ws, counters); break;
-    case 1: run_product<T, false, true, BM>(dp, tile, split, ws, counters); break;
-    case 2: run_product<T, true, false, BM>(dp, tile, split, ws, counters); break;
-    default: run_product<T, true, true, BM>(dp, tile, split, ws, counters); break;
+__device__ __forceinline__ void run_tile(const GroupDesc<T> &d, const WorkUnit &w, T *ws,
+                                         int *counters, int next, int *s_next) {
+  switch (d.ta * 2 + d.tb) {
+    case 0: run_product<T, false, false, BM>(d, w, ws, counters, next, s_next); break;
+    case 1: run_product<T, false, true, BM>(d, w, ws, counters, next, s_next); break;
+    case 2: run_product<T, true, false, BM>(d, w, ws, counters, next, s_next); break;
+    default: run_product<T, true, true, BM>(d, w, ws, counters, next, s_next); break;
   }
 }
 
@@ -101,27 +84,39 @@ __device__ __forceinline__ void run_scale(const GroupDesc<T> &d, const WorkUnit 
   }
 }
 
+// The workgroup claims its next unit as it starts the current one (one
+// atomic in flight under the unit's k loop), so the claim's latency is off
+// the path between units. The index goes through two LDS slots used in
+// turn, and units are separated by a barrier that waits for LDS only: a
+// unit's C stores drain while the next unit loads its descriptor and its
+// first stages.
 template <typename T>
 __global__ __launch_bounds__(kThreads) void grouped_gemm_kernel(const GroupDesc<T> *descs,
                                                                 const WorkUnit *units, int nunits,
                                                                 int *sched, T *ws,
                                                                 int *counters) {
-  __shared__ int s_unit;
-  for (;;) {
-    if (threadIdx.x == 0) s_unit = atomicAdd(&sched[0], 1);
-    __syncthreads();
-    const int u = __builtin_amdgcn_readfirstlane(s_unit);
-    __syncthreads();  // every wave has read s_unit before it is overwritten
-    if (u >= nunits) break;
-    const WorkUnit w = units[u];
-    const GroupDesc<T> *dp = descs + w.prob;
-    const int mode = dp->mode, tile_side = dp->tile, ta = dp->ta, tb = dp->tb;
-    if (mode == 1)
-      run_scale(*dp, w);
-    else if (tile_side == 128)
-      run_tile<T, 128>(dp, ta, tb, w.tile, w.split, ws, counters);
-    else
-      run_tile<T, 64>(dp, ta, tb, w.tile, w.split, ws, counters);
+  __shared__ int s_unit[2];
+  if (threadIdx.x == 0) s_unit[0] = atomicAdd(&sched[0], 1);
+  __syncthreads();
+  int u = __builtin_amdgcn_readfirstlane(s_unit[0]);
+  int slot = 0;
+  while (u < nunits) {
+    int next = 0;
+    if (threadIdx.x == 0) next = atomicAdd(&sched[0], 1);
+    int *s_next = &s_unit[slot ^ 1];
+    const WorkUnit w = const_load(units + u);
+    const GroupDesc<T> d = const_load(descs + w.prob);
+    if (d.mode == 1) {
+      run_scale(d, w);
+      if (threadIdx.x == 0) *s_next = next;
+    } else if (d.tile == 128) {
+      run_tile<T, 128>(d, w, ws, counters, next, s_next);
+    } else {
+      run_tile<T, 64>(d, w, ws, counters, next, s_next);
+    }
+    stage_barrier<63>();
+    slot ^= 1;
+    u = __builtin_amdgcn_readfirstlane(s_unit[slot]);
   }
   // Every workgroup has drawn past the end before it counts itself out, so
   // the last one out can reset the schedule for the next launch.

@@ -2,11 +2,16 @@
 // the grouped persistent GEMM (grouped.hip). Device code only; every
 // translation unit that includes it gets its own LDS stages.
 //
-// One workgroup (8 waves, 2 x 4: two waves per SIMD, so one wave's barrier
-// and LDS waits overlap the other's MFMAs) computes one BM x BN tile of
+// One workgroup (8 waves: two waves per SIMD, so one wave's barrier and LDS
+// waits overlap the other's MFMAs) computes one BM x BN tile of
 // C := alpha * op(A) * op(B) + beta * C (column-major), BM, BN in {64, 128},
-// each wave a (BM/2) x (BN/4) block of 16 x 16 MFMA accumulators, over a
-// k range [kbeg, kend) (the whole k, or one split of it).
+// over a k range [kbeg, kend) (the whole k, or one split of it). Each wave
+// holds (BM/32) x (BN/64) blocks of 16 x 16 MFMA accumulators, interleaved
+// over the tile: wave w (on SIMD w % 4) takes the 16-row blocks w/4,
+// w/4 + 2, ... and the 16-column blocks w%4, w%4 + 4, ... So the blocks a
+// SIMD owns are spread over the whole tile, and on an edge tile, where the
+// blocks past the matrix edge are skipped, every SIMD keeps about the same
+// share of the remaining MFMAs.
 //
 // LDS staging. One DMA instruction moves 64 lanes x 16 bytes = 64*V elements
 // (V = 16 / sizeof(T): 2 for f64, 4 for f32) into one contiguous 1 KiB LDS
@@ -306,14 +311,15 @@ struct TileGemm {
   typedef typename Mfma<T>::acc_t acc_t;
   static constexpr bool A_MN = !TA;   // A NoTrans is M-contiguous
   static constexpr bool B_MN = TB;    // B Trans is N-contiguous
-  static constexpr int FM = BM / 32;  // 16 x 16 blocks per wave along m
-  static constexpr int FN = BN / (16 * (kWaves / 2));  // ... along n
+  static constexpr int FM = BM / 32;  // 16 x 16 blocks per wave along m (row blocks 2i + wh)
+  static constexpr int FN = BN / (16 * (kWaves / 2));  // ... along n (column blocks 4j + wsimd)
   static constexpr int kPerTile = BM / (8 * kWaves) + BN / (8 * kWaves);  // DMA per wave per k-tile
 
   Operand<T, A_MN, BM> a;
   Operand<T, B_MN, BN> b;
   acc_t acc[FM][FN];
-  int kbeg, kend, ktiles, full_tiles, tid, lane, wm, wn;
+  int kbeg, kend, ktiles, full_tiles, tid, lane, wh, wsimd;
+  int nbi, nbj;  // wave-uniform: this wave's row / column blocks that reach into the matrix
 
   // Set up the tile at (m0, n0) over k range [kb, ke) of problem p.
   __device__ __forceinline__ void init(const GemmProblem<T> &p, int m0, int n0, int kb, int ke) {
@@ -326,8 +332,13 @@ struct TileGemm {
     tid = t;
     lane = tid & 63;
     const int wave = tid >> 6;
-    wm = (wave & 1) * (BM / 2);
-    wn = (wave >> 1) * (BN / (kWaves / 2));
+    wh = wave >> 2;
+    wsimd = wave & 3;
+    // Blocks i < nbi (row block 2i + wh) and j < nbj (column block 4j + wsimd)
+    // hold at least one row / column of C; the others are skipped.
+    const int rb = ((int)p.m - m0 + 15) >> 4, cb = ((int)p.n - n0 + 15) >> 4;
+    nbi = __builtin_amdgcn_readfirstlane(max(0, min(FM, (rb - wh + 1) >> 1)));
+    nbj = __builtin_amdgcn_readfirstlane(max(0, min(FN, (cb - wsimd + 3) >> 2)));
     kbeg = kb;
     kend = ke;
     ktiles = (ke - kb + G::BK - 1) / G::BK;
@@ -344,8 +355,7 @@ struct TileGemm {
       for (int j = 0; j < FN; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
   }
 
-  // Stage k-tile kt into the LDS stage (la, lb): DMA when full, masked loads
-  // otherwise.
+  // Stage k-tile kt into the LDS stage (la, lb).
   __device__ __forceinline__ void issue(int kt, char *la, char *lb) {
     if (kt >= ktiles) return;
     const int k0 = kbeg + kt * G::BK;
@@ -361,9 +371,7 @@ struct TileGemm {
   // Barrier after which k-tile kt is in LDS for every wave. The tiles issued
   // after kt are kt+1 .. kt+ahead, kPerTile DMA instructions per wave each;
   // vector memory instructions complete in order, so vmcnt(kPerTile * ahead)
-  // means kt's have landed. (If one of the later tiles was the masked tail,
-  // its register loads were consumed by its own LDS stores, which already
-  // waited for everything issued before them.)
+  // means kt's have landed.
   __device__ __forceinline__ void sync_tile(int kt) {
     const int ahead = min(kStages - 2, ktiles - 1 - kt);
     if (ahead >= 2)
@@ -374,20 +382,31 @@ struct TileGemm {
       stage_barrier<0>();
   }
 
-  // Fragment groups [T0, T1) of the k-tile in stage S.
-  template <int S, int T0 = 0, int T1 = G::NT>
+  // Fragment groups [T0, T1) of the k-tile in stage S. EDGE: only blocks
+  // i < nbi, j < nbj are loaded and multiplied (wave-uniform branches); a
+  // skipped block's accumulators stay zero and reach no stored C entry.
+  template <int S, bool EDGE, int T0 = 0, int T1 = G::NT>
   __device__ __forceinline__ void compute() {
     const char *As = lds_a<S>();
     const char *Bs = lds_b<S>();
     const int g = lane >> 4, c = lane & 15;
     typedef T vec_t __attribute__((ext_vector_type(G::V)));
+    // Opaque copies: the block tests are made here, per step, rather than
+    // hoisted out of the k loop as lane masks that would stay live in SGPRs.
+    int nbi = __builtin_amdgcn_readfirstlane(this->nbi);
+    int nbj = __builtin_amdgcn_readfirstlane(this->nbj);
+    if (EDGE) {
+      asm volatile("" : "+s"(nbi));
+      asm volatile("" : "+s"(nbj));
+    }
 #pragma unroll
     for (int t = T0; t < T1; ++t) {
       T fa[G::V][FM], fb[G::V][FN];
       const int kb = 4 * G::V * t + G::V * g;  // this lane group's first k
 #pragma unroll
       for (int i = 0; i < FM; ++i) {
-        const int mn = wm + 16 * i + c;
+        if (EDGE && i >= nbi) continue;
+        const int mn = row_of(i);
         if constexpr (A_MN) {
 #pragma unroll
           for (int p = 0; p < G::V; ++p)
@@ -400,7 +419,8 @@ struct TileGemm {
       }
 #pragma unroll
       for (int j = 0; j < FN; ++j) {
-        const int mn = wn + 16 * j + c;
+        if (EDGE && j >= nbj) continue;
+        const int mn = 16 * (4 * j + wsimd) + c;
         if constexpr (B_MN) {
 #pragma unroll
           for (int p = 0; p < G::V; ++p)
@@ -417,7 +437,8 @@ struct TileGemm {
         for (int i = 0; i < FM; ++i)
 #pragma unroll
           for (int j = 0; j < FN; ++j)
-            acc[i][j] = Mfma<T>::mma(fb[p][j], fa[p][i], acc[i][j]);
+            if (!EDGE || (i < nbi && j < nbj))
+              acc[i][j] = Mfma<T>::mma(fb[p][j], fa[p][i], acc[i][j]);
     }
   }
 
@@ -426,35 +447,57 @@ struct TileGemm {
   // address work overlaps MFMAs already in flight (+1.4% mean over the eight
   // SGEMM 8192 layouts); f64 issues it right after the barrier, which
   // measured faster there. The k order, and so every result bit, is the same.
-  template <int S>
+  template <int S, bool EDGE>
   __device__ __forceinline__ void step(int kt) {
     if (kt >= ktiles) return;
     sync_tile(kt);  // tile kt visible to all waves; the stage of kt-1 is free
     constexpr int N = (S + kStages - 1) % kStages;
     constexpr int H = sizeof(T) == 4 ? G::NT / 2 : 0;
-    compute<S, 0, H>();
+    compute<S, EDGE, 0, H>();
     issue(kt + kStages - 1, lds_a<N>(), lds_b<N>());
-    compute<S, H, G::NT>();
+    compute<S, EDGE, H, G::NT>();
   }
 
-  // The whole k loop: prologue of 3 stages, then unrolled by the 4 stages.
-  __device__ __forceinline__ void mainloop() {
+  template <bool EDGE>
+  __device__ __forceinline__ void kloop() {
+    for (int kt = 0; kt < ktiles; kt += kStages) {
+      step<0, EDGE>(kt);
+      step<1, EDGE>(kt + 1);
+      step<2, EDGE>(kt + 2);
+      step<3, EDGE>(kt + 3);
+    }
+  }
+
+  // The first 3 stages of the pipeline.
+  __device__ __forceinline__ void prologue() {
     issue(0, lds_a<0>(), lds_b<0>());
     issue(1, lds_a<1>(), lds_b<1>());
     issue(2, lds_a<2>(), lds_b<2>());
-    for (int kt = 0; kt < ktiles; kt += kStages) {
-      step<0>(kt);
-      step<1>(kt + 1);
-      step<2>(kt + 2);
-      step<3>(kt + 3);
-    }
+  }
+
+  // The k loop after the prologue, unrolled by the 4 stages. A wave whose
+  // blocks all lie inside the matrix runs the branch-free loop; on an edge
+  // tile the others run the one that skips blocks past the edge (same
+  // barriers and DMA, so the waves stay in step).
+  __device__ __forceinline__ void kloops() {
+    if (nbi == FM && nbj == FN)
+      kloop<false>();
+    else
+      kloop<true>();
+  }
+
+  __device__ __forceinline__ void mainloop() {
+    prologue();
+    kloops();
   }
 
   // Row of C (within the tile) held by this lane in block row i; column of
   // element r of block column j.
-  __device__ __forceinline__ int row_of(int i) const { return wm + 16 * i + (lane & 15); }
+  __device__ __forceinline__ int row_of(int i) const {
+    return 16 * (2 * i + wh) + (lane & 15);
+  }
   __device__ __forceinline__ int col_of(int j, int r) const {
-    return wn + 16 * j + Mfma<T>::row(lane, r);
+    return 16 * (4 * j + wsimd) + Mfma<T>::row(lane, r);
   }
 
   // C := alpha * acc + beta * C on the tile at (m0, n0); beta == 0 never
