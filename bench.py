@@ -44,6 +44,7 @@ trace and the f64 Level-1 calls (full runs only).
 from __future__ import annotations
 
 import argparse
+import functools
 import json
 import os
 import sys
@@ -69,6 +70,8 @@ def parse():
     p.add_argument("--quick", action="store_true", help="headline DGEMM only")
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--serve-requests", type=int, default=10000)
+    p.add_argument("--serve-inflight", type=int, default=4,
+                   help="batches in flight for --only serve_batched (profiling)")
     p.add_argument("--only", choices=sorted(ONLY),
                    help="run one component and print its JSON (for profiling)")
     p.add_argument("--copy-calibrate", action="store_true",
@@ -521,29 +524,36 @@ def level2_bw(n=16384, steps=10):
     return out
 
 
+# The serving runs fingerprint every 8th column of each result (the tests
+# hash every column): the hash is the harness's record of a result, not
+# serving work.
+SERVE_HASH_COL_STEP = 8
+
+
 def serving(count):
     """Returns (summary, the timed request trace)."""
     from rblas_amd import serve
+    run = functools.partial(serve.run, hash_col_step=SERVE_HASH_COL_STEP)
     closed = serve.workload(count, 0.0)
-    res, st = serve.run(closed, serve.BATCHED)
+    res, st = run(closed, serve.BATCHED)
     capacity = count / st["wall_s"]
     rate = 0.7 * capacity
     trace = serve.workload(count, rate)
-    res, st = serve.run(trace, serve.BATCHED)
+    res, st = run(trace, serve.BATCHED)
     batched = serve.summary(res, st)
-    res2, st2 = serve.run(trace, serve.PLAIN)
+    res2, st2 = run(trace, serve.PLAIN)
     plain = serve.summary(res2, st2)
     # The plain per-call path cannot sustain the batched loop's rate (its
     # queue grows without bound there); it is also run at 70% of its own
     # closed-loop capacity, a rate it sustains, for a p99-vs-p99 comparison.
-    _, stc = serve.run(closed, serve.PLAIN)
+    _, stc = run(closed, serve.PLAIN)
     plain_capacity = count / stc["wall_s"]
     plain_rate = 0.7 * plain_capacity
-    res3, st3 = serve.run(serve.workload(count, plain_rate), serve.PLAIN)
+    res3, st3 = run(serve.workload(count, plain_rate), serve.PLAIN)
     plain_own = serve.summary(res3, st3)
     keys = ("tflops", "p50_ms", "p99_ms", "wall_s", "batches", "peak_footprint_gb")
     return {"requests": count, "rate_rps": round(rate, 1),
-            "capacity_rps": round(capacity, 1),
+            "capacity_rps": round(capacity, 1), "hash_col_step": SERVE_HASH_COL_STEP,
             "batched": {k: round(batched[k], 3) for k in keys},
             "plain": {k: round(plain[k], 3) for k in keys},
             "plain_capacity_rps": round(plain_capacity, 1),
@@ -551,13 +561,17 @@ def serving(count):
                                        **{k: round(plain_own[k], 3) for k in keys})}, trace
 
 
-def serving_batched(count):
+def serving_batched(count, inflight=4):
     """The continuous-batching runs of serving() alone (closed loop, then
-    Poisson at 70% of its capacity), for profiling the batched kernel mix."""
+    Poisson at 70% of its capacity), for profiling the batched kernel mix;
+    inflight 1 runs one batch at a time, so each kernel's trace duration is
+    its own execution time rather than time shared with other batches."""
     from rblas_amd import serve
-    res, st = serve.run(serve.workload(count, 0.0), serve.BATCHED)
+    run = functools.partial(serve.run, hash_col_step=SERVE_HASH_COL_STEP,
+                            max_inflight=inflight)
+    res, st = run(serve.workload(count, 0.0), serve.BATCHED)
     rate = 0.7 * count / st["wall_s"]
-    res, st = serve.run(serve.workload(count, rate), serve.BATCHED)
+    res, st = run(serve.workload(count, rate), serve.BATCHED)
     out = serve.summary(res, st)
     out["rate_rps"] = rate
     return {k: round(v, 3) for k, v in out.items()}
@@ -728,7 +742,7 @@ ONLY = {
     "ctrsm": lambda a: ctrsm_single(),
     "level2": lambda a: level2_bw(),
     "serve": lambda a: serving(a.serve_requests)[0],
-    "serve_batched": lambda a: serving_batched(a.serve_requests),
+    "serve_batched": lambda a: serving_batched(a.serve_requests, a.serve_inflight),
     "c64_gemm": lambda a: c64_gemm(),
     "sgemm_skinny": lambda a: skinny_sgemm_tflops(),
     "level1": lambda a: level1_bw(),

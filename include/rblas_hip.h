@@ -626,10 +626,18 @@ int32_t rblas_grouped_get_tuning(int32_t dtype, double *coeffs);
  * non-decreasing), joins a FIFO queue, is admitted when a stream slot is free
  * and the footprint of all admitted, unfinished requests stays within
  * hbm_cap_bytes, and completes when its batch's work has finished. Operands
- * are synthetic, generated in HBM from `seed` at launch (workloads.py's
- * counter-based splitmix64, draw order A, B, C; uniform(-1, 1); a trsm A is
- * lower triangular: off-diagonal u / sqrt(m), diagonal 2 + |u|) and freed on
- * completion. Kinds (column-major, NoTrans):
+ * are synthetic (workloads.py's counter-based splitmix64, uniform(-1, 1)).
+ * The read-only A and B of gemm and complex requests are slices of an
+ * operand pool per element type, generated in HBM once before the first
+ * arrival: pool element i is draw i of seed RBLAS_SERVE_POOL_SEED, and the
+ * slice of operand j (A: 1, B: 2) starts at element
+ * mix(seed ^ j) mod (pool - len + 1), mix the splitmix64 finalizer, the pool
+ * holding max(2^25, the largest such operand) elements. Written operands are
+ * generated in HBM from `seed` at launch and freed on completion: gemm and
+ * complex C from offset len(A) + len(B); a trsm A (lower triangular:
+ * off-diagonal u / sqrt(m), diagonal 2 + |u|, zeros above) from 0 and B from
+ * m * m. A request's footprint counts its written operands and workspace.
+ * Kinds (column-major, NoTrans):
  *   RBLAS_SERVE_GEMM   C (m x n) := 1 * A (m x k) B (k x n) + 0.5 C, dtype 0 f32 / 1 f64
  *   RBLAS_SERVE_TRSM   A (m x m) X = B (m x n), Left, Lower, NonUnit, alpha 1
  *   RBLAS_SERVE_ZGEMM  complex f64: C := (1 + 0i) A B + (0.5 + 0i) C (4M)
@@ -640,12 +648,13 @@ int32_t rblas_grouped_get_tuning(int32_t dtype, double *coeffs);
  * request at a time in FIFO order through the single-call kernels.
  * results[i] gets the request's times (seconds from the start), algorithmic
  * flops (2mnk, m^2 n, 8mnk), footprint and an order-independent 64-bit hash
- * of the result's bits; a request's hash does not depend on what else ran
- * (same batch mode). capture_ids / capture_out (optional, tests): the dense
+ * of the result's bits (of every hash_col_step-th column); a request's hash
+ * does not depend on what else ran (same batch mode). capture_ids / capture_out (optional, tests): the dense
  * result of those requests is copied to the given host buffers. Blocks until
  * every request has completed. */
 enum { RBLAS_SERVE_GEMM = 0, RBLAS_SERVE_TRSM = 1, RBLAS_SERVE_ZGEMM = 2 };
 enum { RBLAS_SERVE_BATCHED = 0, RBLAS_SERVE_PLAIN = 1 };
+#define RBLAS_SERVE_POOL_SEED 0x5EEDB10C5EEDB10Cull
 
 typedef struct {
   int32_t kind;
@@ -661,6 +670,9 @@ typedef struct {
   int32_t max_inflight; /* batches on the device at once (batched mode)      */
   int32_t max_batch;    /* requests per batch (batched mode)                 */
   int32_t capture_count;
+  int32_t hash_col_step; /* result columns hashed: every one (0 or 1) or
+                            every hash_col_step-th                           */
+  int32_t pad;
   double hbm_cap_bytes; /* admission cap on admitted, unfinished footprints  */
   const int32_t *capture_ids;
   void *const *capture_out;

@@ -73,7 +73,8 @@ class GroupedInfo(C.Structure):
 class ServeConfig(C.Structure):
     """`rblas_serve_config_t`."""
     _fields_ = [("mode", C.c_int32), ("max_inflight", C.c_int32), ("max_batch", C.c_int32),
-                ("capture_count", C.c_int32), ("hbm_cap_bytes", C.c_double),
+                ("capture_count", C.c_int32), ("hash_col_step", C.c_int32), ("pad", C.c_int32),
+                ("hbm_cap_bytes", C.c_double),
                 ("capture_ids", C.POINTER(C.c_int32)), ("capture_out", C.POINTER(C.c_void_p))]
 
 

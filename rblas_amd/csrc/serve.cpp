@@ -2,9 +2,11 @@
 // against the wall clock, with admission by HBM footprint and continuous
 // batching (no rblas counterpart: rblas is a library of calls).
 //
-// Each request's operands are generated in HBM at launch (serve.hip) and
-// freed when its batch completes; its result is hashed on the device. The
-// scheduler (the caller's thread) loops:
+// The read-only A and B of gemm and complex requests are slices of an operand
+// pool generated in HBM before the first arrival (resident data, as a
+// server's weights would be); each request's written operands are generated
+// in HBM at launch (serve.hip) and freed when its batch completes; its
+// result is hashed on the device. The scheduler (the caller's thread) loops:
 //   1. requests whose arrival time has passed join a FIFO queue;
 //   2. finished batches (their event has completed) retire: their requests
 //      complete, their footprint is released;
@@ -45,10 +47,25 @@ double seconds_since(Clock::time_point t0) {
 struct Req {
   rblas_serve_request_t spec;
   double footprint = 0, flops = 0;
-  size_t bytes[4] = {0, 0, 0, 0};  // A, B, C, workspace (device bytes)
+  size_t bytes[4] = {0, 0, 0, 0};  // A, B, C, workspace (device bytes of the request's own)
   size_t off[4] = {0, 0, 0, 0};
-  int esize = 8;                    // real element size of the operands
+  const char *pooled[2] = {nullptr, nullptr};  // A, B as operand-pool slices (gemm, complex)
+  int64_t pool_len[2] = {0, 0};                // their lengths in pool elements
+  int esize = 8;                               // real element size of the operands
 };
+
+// The read-only operand pools (one per element type) that gemm and complex
+// requests take A and B from: element i is draw i of RBLAS_SERVE_POOL_SEED.
+struct Pools {
+  char *base[2] = {nullptr, nullptr};  // f32, f64
+  int64_t elems[2] = {0, 0};
+};
+
+// Element offset of operand j (1: A, 2: B) of length len in a pool of
+// `elems` elements, for a request drawn from `seed`.
+int64_t pool_offset(uint64_t seed, int j, int64_t len, int64_t elems) {
+  return (int64_t)(mix64(seed ^ (uint64_t)j) % (uint64_t)(elems - len + 1));
+}
 
 // A stream slot: one batch in flight at a time. Its device arena holds the
 // batch's operands, tables and workspaces; it is sized before the run starts
@@ -99,8 +116,8 @@ Req describe(const rblas_serve_request_t &s) {
   const double m = s.m, n = s.n, k = s.k;
   if (s.kind == RBLAS_SERVE_GEMM) {
     r.esize = s.dtype == 0 ? 4 : 8;
-    r.bytes[0] = (size_t)s.m * s.k * r.esize;
-    r.bytes[1] = (size_t)s.k * s.n * r.esize;
+    r.pool_len[0] = (int64_t)s.m * s.k;
+    r.pool_len[1] = (int64_t)s.k * s.n;
     r.bytes[2] = (size_t)s.m * s.n * r.esize;
     r.flops = 2 * m * n * k;
   } else if (s.kind == RBLAS_SERVE_TRSM) {
@@ -110,8 +127,8 @@ Req describe(const rblas_serve_request_t &s) {
     r.flops = m * m * n;
   } else {
     r.esize = 8;  // complex f64 as interleaved doubles
-    r.bytes[0] = (size_t)s.m * s.k * 16;
-    r.bytes[1] = (size_t)s.k * s.n * 16;
+    r.pool_len[0] = 2 * (int64_t)s.m * s.k;
+    r.pool_len[1] = 2 * (int64_t)s.k * s.n;
     r.bytes[2] = (size_t)s.m * s.n * 16;
     r.bytes[3] = cgemm_workspace_elems(s.m, s.n, s.k) * sizeof(double);
     r.flops = 8 * m * n * k;
@@ -120,8 +137,9 @@ Req describe(const rblas_serve_request_t &s) {
   return r;
 }
 
-// Seeds: operand j of request with seed s draws from offsets in the order
-// A, B, C (workloads.py's draw order), each from offset = elements before it.
+// Seeds: the written operands of a request with seed s draw from offsets
+// in the order A, B, C (workloads.py's draw order), each from offset =
+// elements before it (pooled A and B counted).
 struct Planner {
   const rblas_serve_config_t &cfg;
   std::vector<Req> &reqs;
@@ -181,30 +199,38 @@ struct Planner {
       gens.push_back(GenDesc{p, count, n, seed, offset, dtype, mode});
       for (int64_t c = 0; c * kGenChunk < count; ++c) gen_units.push_back(int2{d, (int)c});
     };
+    const int col_step = std::max(1, cfg.hash_col_step);
     auto add_hash = [&](const void *p, int64_t rows, int64_t cols, int eb) {
       const int d = (int)hashes.size();
-      hashes.push_back(HashDesc{p, rows, cols, rows, eb, 0});
-      for (int64_t c = 0; c * kGenChunk < rows * cols; ++c) hash_units.push_back(int2{d, (int)c});
+      hashes.push_back(HashDesc{p, rows, cols, rows, eb, col_step});
+      for (int64_t c = 0; c * kGenChunk < hash_elems(hashes.back()); ++c)
+        hash_units.push_back(int2{d, (int)c});
     };
+    // Group descriptors whose A and B are pool slices (absolute addresses,
+    // not rebased onto the arena).
+    std::vector<char> g32_abs, g64_abs;
     for (int id : ids) {
       const Req &r = reqs[id];
       const rblas_serve_request_t &s = r.spec;
       char *a = ops + r.off[0], *b = ops + r.off[1], *c = ops + r.off[2];
       if (s.kind == RBLAS_SERVE_GEMM) {
         const int64_t la = (int64_t)s.m * s.k, lb = (int64_t)s.k * s.n, lc = (int64_t)s.m * s.n;
-        add_gen(a, la, s.dtype, 0, 0, s.seed, 0);
-        add_gen(b, lb, s.dtype, 0, 0, s.seed, la);
         add_gen(c, lc, s.dtype, 0, 0, s.seed, la + lb);
         add_hash(c, s.m, s.n, r.esize);
         if (!plain) {
-          if (s.dtype == 0)
-            g32.add(GemmProblem<float>{false, false, s.m, s.n, s.k, 1.0f, 0.5f, (float *)a, s.m,
-                                       (float *)b, s.k, (float *)c, s.m},
+          if (s.dtype == 0) {
+            g32.add(GemmProblem<float>{false, false, s.m, s.n, s.k, 1.0f, 0.5f,
+                                       (const float *)r.pooled[0], s.m,
+                                       (const float *)r.pooled[1], s.k, (float *)c, s.m},
                     tu);
-          else
-            g64.add(GemmProblem<double>{false, false, s.m, s.n, s.k, 1.0, 0.5, (double *)a, s.m,
-                                        (double *)b, s.k, (double *)c, s.m},
+            g32_abs.push_back(1);
+          } else {
+            g64.add(GemmProblem<double>{false, false, s.m, s.n, s.k, 1.0, 0.5,
+                                        (const double *)r.pooled[0], s.m,
+                                        (const double *)r.pooled[1], s.k, (double *)c, s.m},
                     tu);
+            g64_abs.push_back(1);
+          }
         }
       } else if (s.kind == RBLAS_SERVE_TRSM) {
         const int64_t la = (int64_t)s.m * s.m, lb = (int64_t)s.m * s.n;
@@ -222,8 +248,6 @@ struct Planner {
       } else {
         const int64_t la = 2 * (int64_t)s.m * s.k, lb = 2 * (int64_t)s.k * s.n;
         const int64_t lc = 2 * (int64_t)s.m * s.n;
-        add_gen(a, la, 1, 0, 0, s.seed, 0);
-        add_gen(b, lb, 1, 0, 0, s.seed, la);
         add_gen(c, lc, 1, 0, 0, s.seed, la + lb);
         add_hash(c, 2 * (int64_t)s.m, s.n, 8);
         if (!plain) {
@@ -235,8 +259,8 @@ struct Planner {
           double *ap = ws, *b1 = ap + 2 * m * k, *b2 = b1 + 2 * k * n, *pr = b2 + 2 * k * n;
           double *pi = pr + m * n;
           const int d = (int)cplx.size();
-          cplx.push_back(CplxDesc{(const double *)a, (const double *)b, (double *)c, ws, m, n, k,
-                                  1.0, 0.0, 0.5, 0.0});
+          cplx.push_back(CplxDesc{(const double *)r.pooled[0], (const double *)r.pooled[1],
+                                  (double *)c, ws, m, n, k, 1.0, 0.0, 0.5, 0.0});
           const int64_t chunks = (m * k + kGenChunk - 1) / kGenChunk +
                                  (k * n + kGenChunk - 1) / kGenChunk;
           for (int64_t ch = 0; ch < chunks; ++ch) cplx_split_units.push_back(int2{d, (int)ch});
@@ -248,6 +272,8 @@ struct Planner {
           g64.add(GemmProblem<double>{false, false, m, n, 2 * k, 1.0, 0.0, ap, m, b2, 2 * k, pi,
                                       m},
                   tu);
+          g64_abs.push_back(0);
+          g64_abs.push_back(0);
         }
       }
     }
@@ -287,13 +313,21 @@ struct Planner {
     if ((e = ensure_arena(slot, ops_bytes + dev_bytes)) != hipSuccess) return e;
     ops = slot.arena;
     auto rebase = [&](auto *p) { return (decltype(p))((char *)p + (uintptr_t)ops); };
-    for (auto &x : g32.descs) x.a = rebase(x.a), x.b = rebase(x.b), x.c = rebase(x.c);
-    for (auto &x : g64.descs) x.a = rebase(x.a), x.b = rebase(x.b), x.c = rebase(x.c);
+    for (size_t i = 0; i < g32.descs.size(); ++i) {
+      auto &x = g32.descs[i];
+      if (!g32_abs[i]) x.a = rebase(x.a), x.b = rebase(x.b);
+      x.c = rebase(x.c);
+    }
+    for (size_t i = 0; i < g64.descs.size(); ++i) {
+      auto &x = g64.descs[i];
+      if (!g64_abs[i]) x.a = rebase(x.a), x.b = rebase(x.b);
+      x.c = rebase(x.c);
+    }
     for (auto &x : t32.descs) x.a = rebase(x.a), x.b = rebase(x.b);
     for (auto &x : t64.descs) x.a = rebase(x.a), x.b = rebase(x.b);
     for (auto &x : gens) x.ptr = rebase((char *)x.ptr);
     for (auto &x : hashes) x.ptr = rebase((const char *)x.ptr);
-    for (auto &x : cplx) x.a = rebase(x.a), x.b = rebase(x.b), x.c = rebase(x.c), x.ws = rebase(x.ws);
+    for (auto &x : cplx) x.c = rebase(x.c), x.ws = rebase(x.ws);  // a, b: pool slices
     char *h = slot.pinned;
     std::memcpy(h + o_gd, gens.data(), gens.size() * sizeof(GenDesc));
     std::memcpy(h + o_gu, gen_units.data(), gen_units.size() * sizeof(int2));
@@ -334,7 +368,9 @@ struct Planner {
     for (size_t i = 0; i < ids.size(); ++i) {
       const Req &r = reqs[ids[i]];
       const rblas_serve_request_t &s = r.spec;
-      char *a = ops + r.off[0], *b = ops + r.off[1], *c = ops + r.off[2];
+      const char *a = r.pooled[0] ? r.pooled[0] : ops + r.off[0];
+      const char *b = r.pooled[1] ? r.pooled[1] : ops + r.off[1];
+      char *c = ops + r.off[2];
       if (plain && s.kind == RBLAS_SERVE_ZGEMM) {
         CgemmProblem<double> p{0, 0, s.m, s.n, s.k, 1.0, 0.0, 0.5, 0.0, (const double *)a, s.m,
                                (const double *)b, s.k, (double *)c, s.m};
@@ -414,6 +450,7 @@ extern "C" int32_t rblas_serve_run(const rblas_serve_request_t *reqs_in, size_t 
   const int max_batch = plain ? 1 : std::max(1, cfg->max_batch);
   if (cfg->capture_count < 0 || (cfg->capture_count > 0 && !cfg->capture_ids))
     return fail(bad_arg("serve: bad capture list"), nullptr);
+  if (cfg->hash_col_step < 0) return fail(bad_arg("serve: negative hash_col_step"), nullptr);
   std::vector<Req> reqs(count);
   for (size_t i = 0; i < count; ++i) {
     const rblas_serve_request_t &s = reqs_in[i];
@@ -440,6 +477,54 @@ extern "C" int32_t rblas_serve_run(const rblas_serve_request_t *reqs_in, size_t 
       sum += reqs[j].footprint;
     }
     window = std::max(window, sum);
+  }
+  // Operand pools, generated before the first arrival.
+  struct PoolGuard {
+    Pools p;
+    ~PoolGuard() {
+      for (char *b : p.base)
+        if (b) (void)hipFree(b);
+    }
+  } pool;
+  {
+    int64_t need[2] = {0, 0};
+    for (const Req &r : reqs) {
+      const int dt = r.spec.kind == RBLAS_SERVE_ZGEMM ? 1 : r.spec.dtype;
+      need[dt] = std::max(need[dt], std::max(r.pool_len[0], r.pool_len[1]));
+    }
+    std::vector<GenDesc> gd;
+    std::vector<int2> gu;
+    for (int dt = 0; dt < 2; ++dt) {
+      if (need[dt] == 0) continue;
+      const int64_t elems = std::max<int64_t>(int64_t(1) << 25, need[dt]);
+      RBLAS_HIP_TRY(hipMalloc((void **)&pool.p.base[dt], (size_t)elems * (dt ? 8 : 4)),
+                    "serve operand pool");
+      pool.p.elems[dt] = elems;
+      const int d = (int)gd.size();
+      gd.push_back(GenDesc{pool.p.base[dt], elems, 0, RBLAS_SERVE_POOL_SEED, 0, dt, 0});
+      for (int64_t c = 0; c * kGenChunk < elems; ++c) gu.push_back(int2{d, (int)c});
+    }
+    if (!gd.empty()) {
+      const size_t bd = round256(gd.size() * sizeof(GenDesc)), bu = gu.size() * sizeof(int2);
+      char *tab = nullptr;
+      RBLAS_HIP_TRY(hipMalloc((void **)&tab, bd + bu), "serve pool tables");
+      hipError_t e = hipMemcpy(tab, gd.data(), gd.size() * sizeof(GenDesc), hipMemcpyHostToDevice);
+      if (e == hipSuccess) e = hipMemcpy(tab + bd, gu.data(), bu, hipMemcpyHostToDevice);
+      if (e == hipSuccess) e = launch_gen((const GenDesc *)tab, (const int2 *)(tab + bd),
+                                          (int)gu.size(), nullptr);
+      if (e == hipSuccess) e = hipDeviceSynchronize();
+      const hipError_t f = hipFree(tab);
+      if (e == hipSuccess) e = f;
+      RBLAS_HIP_TRY(e, "serve pool generation");
+    }
+    for (Req &r : reqs) {
+      if (r.pool_len[0] == 0) continue;
+      const int dt = r.spec.kind == RBLAS_SERVE_ZGEMM ? 1 : r.spec.dtype;
+      const size_t es = dt ? 8 : 4;
+      for (int j = 0; j < 2; ++j)
+        r.pooled[j] = pool.p.base[dt] +
+                      es * (size_t)pool_offset(r.spec.seed, j + 1, r.pool_len[j], pool.p.elems[dt]);
+    }
   }
   std::vector<Slot> slots(slots_n);
   for (Slot &s : slots) {

@@ -10,8 +10,9 @@
 // above.
 //
 // The hash of a rows x cols column-major block is the sum mod 2^64 over its
-// elements of mix(bits(x_e) ^ (e * golden)), e the column-major index: a sum,
-// so the order in which blocks add their parts does not matter.
+// elements (or those of every col_step-th column) of
+// mix(bits(x_e) ^ (e * golden)), e the column-major index: a sum, so the
+// order in which blocks add their parts does not matter.
 #include "serve.hpp"
 
 namespace rblas {
@@ -19,11 +20,7 @@ namespace {
 
 constexpr uint64_t kGolden = 0x9E3779B97F4A7C15ull;
 
-__device__ __forceinline__ uint64_t mix(uint64_t z) {
-  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
-  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
-  return z ^ (z >> 31);
-}
+__device__ __forceinline__ uint64_t mix(uint64_t z) { return mix64(z); }
 
 template <typename T>
 __device__ __forceinline__ T uniform_at(uint64_t seed, int64_t idx);
@@ -90,38 +87,39 @@ __global__ __launch_bounds__(256) void gen_kernel(const GenDesc *descs, const in
     gen_range<double>(d, lo, hi);
 }
 
+// Each thread hashes kHashPer elements (indices lo + threadIdx.x + 256 t):
+// all their loads are issued before any is used, so a thread waits for
+// memory once, not once per element. Element e of the visited sequence is
+// row e % rows of column col_step * (e / rows) (32-bit division: a hashed
+// block holds fewer than 2^31 elements).
+constexpr int kHashPer = kGenChunk / 256;
+
 __global__ __launch_bounds__(256) void hash_kernel(const HashDesc *descs, const int2 *units,
                                                    unsigned long long *out) {
   __shared__ unsigned long long part[256];
   const int2 u = units[blockIdx.x];
   const HashDesc d = descs[u.x];
-  const int64_t total = d.rows * d.cols;
+  const int64_t total = hash_elems(d);
   const int64_t lo = (int64_t)u.y * kGenChunk;
-  const int64_t hi = min(total, lo + (int64_t)kGenChunk);
-  unsigned long long acc = 0;
-  const int64_t e0 = lo + threadIdx.x;
-  if (d.ld == d.rows) {  // dense block: the element of index e is at e
-    if (d.elem_bytes == 4) {
-      const uint32_t *x = (const uint32_t *)d.ptr;
-      for (int64_t e = e0; e < hi; e += blockDim.x) acc += mix(x[e] ^ ((uint64_t)e * kGolden));
-    } else {
-      const uint64_t *x = (const uint64_t *)d.ptr;
-      for (int64_t e = e0; e < hi; e += blockDim.x) acc += mix(x[e] ^ ((uint64_t)e * kGolden));
-    }
-  } else {
-    int64_t i, j;
-    split_index(e0, d.rows, i, j);
-    for (int64_t e = e0; e < hi; e += blockDim.x) {
-      const int64_t at = i + j * d.ld;
-      uint64_t bits;
-      if (d.elem_bytes == 4)
-        bits = ((const uint32_t *)d.ptr)[at];
-      else
-        bits = ((const uint64_t *)d.ptr)[at];
-      acc += mix(bits ^ ((uint64_t)e * kGolden));
-      advance(blockDim.x, d.rows, i, j);
-    }
+  const uint32_t rows = (uint32_t)d.rows;
+  uint64_t bits[kHashPer];
+  uint64_t idx[kHashPer];
+#pragma unroll
+  for (int t = 0; t < kHashPer; ++t) {
+    const int64_t e = lo + threadIdx.x + 256 * t;
+    const uint32_t jj = (uint32_t)e / rows, i = (uint32_t)e - jj * rows;
+    const int64_t j = (int64_t)jj * d.col_step;
+    const int64_t at = e < total ? i + j * d.ld : 0;
+    idx[t] = e < total ? (uint64_t)(i + j * d.rows) : ~0ull;
+    if (d.elem_bytes == 4)
+      bits[t] = ((const uint32_t *)d.ptr)[at];
+    else
+      bits[t] = ((const uint64_t *)d.ptr)[at];
   }
+  unsigned long long acc = 0;
+#pragma unroll
+  for (int t = 0; t < kHashPer; ++t)
+    if (idx[t] != ~0ull) acc += mix(bits[t] ^ (idx[t] * kGolden));
   part[threadIdx.x] = acc;
   __syncthreads();
   for (int s = 128; s > 0; s >>= 1) {
@@ -143,6 +141,12 @@ __device__ __forceinline__ int64_t a_chunks(const CplxDesc &d) {
   return (d.m * d.k + kGenChunk - 1) / kGenChunk;
 }
 
+// The two kernels below move kGenChunk elements per workgroup in batches of
+// kCplxBatch per thread, all of a batch's loads issued before its stores: a
+// load after a store to a buffer the compiler cannot tell apart would
+// otherwise wait for memory once per element.
+constexpr int kCplxBatch = 8;
+
 __global__ __launch_bounds__(256) void cplx_split_kernel(const CplxDesc *descs,
                                                          const int2 *units) {
   const int2 u = units[blockIdx.x];
@@ -152,25 +156,48 @@ __global__ __launch_bounds__(256) void cplx_split_kernel(const CplxDesc *descs,
   const int64_t ca = a_chunks(d);
   if (u.y < ca) {  // [Ar | Ai]: entry e = i + p m of op(A)
     const int64_t lo = (int64_t)u.y * kGenChunk, hi = min(mk, lo + (int64_t)kGenChunk);
-    for (int64_t e = lo + threadIdx.x; e < hi; e += blockDim.x) {
-      ap[e] = d.a[2 * e];
-      ap[e + mk] = d.a[2 * e + 1];
+    for (int64_t b0 = lo + threadIdx.x; b0 < hi; b0 += 256 * kCplxBatch) {
+      double re[kCplxBatch], im[kCplxBatch];
+#pragma unroll
+      for (int t = 0; t < kCplxBatch; ++t) {
+        const int64_t e = min(b0 + 256 * t, hi - 1);
+        re[t] = d.a[2 * e];
+        im[t] = d.a[2 * e + 1];
+      }
+#pragma unroll
+      for (int t = 0; t < kCplxBatch; ++t) {
+        const int64_t e = b0 + 256 * t;
+        if (e < hi) {
+          ap[e] = re[t];
+          ap[e + mk] = im[t];
+        }
+      }
     }
     return;
   }
   // [Br; -Bi] and [Bi; Br] (2k x n, ld 2k): entry e = p + j k of op(B)
+  // (32-bit division: k n < 2^31).
   const int64_t lo = (u.y - ca) * kGenChunk, hi = min(kn, lo + (int64_t)kGenChunk);
-  const int64_t e0 = lo + threadIdx.x;
-  int64_t p, j;
-  split_index(e0, d.k, p, j);
-  for (int64_t e = e0; e < hi; e += blockDim.x) {
-    const double re = d.b[2 * e], im = d.b[2 * e + 1];
-    const int64_t col = j * 2 * d.k;
-    b1[col + p] = re;
-    b1[col + d.k + p] = -im;
-    b2[col + p] = im;
-    b2[col + d.k + p] = re;
-    advance(blockDim.x, d.k, p, j);
+  const uint32_t k = (uint32_t)d.k;
+  for (int64_t b0 = lo + threadIdx.x; b0 < hi; b0 += 256 * kCplxBatch) {
+    double re[kCplxBatch], im[kCplxBatch];
+#pragma unroll
+    for (int t = 0; t < kCplxBatch; ++t) {
+      const int64_t e = min(b0 + 256 * t, hi - 1);
+      re[t] = d.b[2 * e];
+      im[t] = d.b[2 * e + 1];
+    }
+#pragma unroll
+    for (int t = 0; t < kCplxBatch; ++t) {
+      const int64_t e = b0 + 256 * t;
+      if (e >= hi) continue;
+      const uint32_t j = (uint32_t)e / k, p = (uint32_t)e - j * k;
+      const int64_t col = (int64_t)j * 2 * d.k;
+      b1[col + p] = re[t];
+      b1[col + d.k + p] = -im[t];
+      b2[col + p] = im[t];
+      b2[col + d.k + p] = re[t];
+    }
   }
 }
 
@@ -182,19 +209,30 @@ __global__ __launch_bounds__(256) void cplx_combine_kernel(const CplxDesc *descs
   const double *pr = d.ws + 2 * mk + 4 * kn, *pi = pr + mn;
   const bool beta_zero = d.beta_re == 0.0 && d.beta_im == 0.0;
   const int64_t lo = (int64_t)u.y * kGenChunk, hi = min(mn, lo + (int64_t)kGenChunk);
-  for (int64_t e = lo + threadIdx.x; e < hi; e += blockDim.x) {
-    const double p_re = pr[e], p_im = pi[e];
-    double s_re = 0.0, s_im = 0.0;
-    double *cp = d.c + 2 * e;
-    if (!beta_zero) {
-      const double c_re = cp[0], c_im = cp[1];
-      s_re = __builtin_fma(d.beta_re, c_re, -(d.beta_im * c_im));
-      s_im = __builtin_fma(d.beta_re, c_im, d.beta_im * c_re);
+  for (int64_t b0 = lo + threadIdx.x; b0 < hi; b0 += 256 * kCplxBatch) {
+    double p_re[kCplxBatch], p_im[kCplxBatch], c_re[kCplxBatch], c_im[kCplxBatch];
+#pragma unroll
+    for (int t = 0; t < kCplxBatch; ++t) {
+      const int64_t e = min(b0 + 256 * t, hi - 1);
+      p_re[t] = pr[e];
+      p_im[t] = pi[e];
+      c_re[t] = beta_zero ? 0.0 : d.c[2 * e];
+      c_im[t] = beta_zero ? 0.0 : d.c[2 * e + 1];
     }
-    const double q_re = __builtin_fma(d.alpha_re, p_re, -(d.alpha_im * p_im));
-    const double q_im = __builtin_fma(d.alpha_re, p_im, d.alpha_im * p_re);
-    cp[0] = q_re + s_re;
-    cp[1] = q_im + s_im;
+#pragma unroll
+    for (int t = 0; t < kCplxBatch; ++t) {
+      const int64_t e = b0 + 256 * t;
+      if (e >= hi) continue;
+      double s_re = 0.0, s_im = 0.0;
+      if (!beta_zero) {
+        s_re = __builtin_fma(d.beta_re, c_re[t], -(d.beta_im * c_im[t]));
+        s_im = __builtin_fma(d.beta_re, c_im[t], d.beta_im * c_re[t]);
+      }
+      const double q_re = __builtin_fma(d.alpha_re, p_re[t], -(d.alpha_im * p_im[t]));
+      const double q_im = __builtin_fma(d.alpha_re, p_im[t], d.alpha_im * p_re[t]);
+      d.c[2 * e] = q_re + s_re;
+      d.c[2 * e + 1] = q_im + s_im;
+    }
   }
 }
 

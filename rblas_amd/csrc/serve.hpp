@@ -19,13 +19,26 @@ struct GenDesc {
   int mode;
 };
 
-// Hash of a rows x cols column-major block with leading dimension ld.
+// Hash of a rows x cols column-major block with leading dimension ld, over
+// its columns 0, col_step, 2 col_step, ...
 struct HashDesc {
   const void *ptr;
   int64_t rows, cols, ld;
   int elem_bytes;  // 4 or 8
-  int pad;
+  int col_step;    // >= 1
 };
+
+// Elements the hash of d visits.
+__host__ __device__ inline int64_t hash_elems(const HashDesc &d) {
+  return d.rows * ((d.cols + d.col_step - 1) / d.col_step);
+}
+
+// The splitmix64 finalizer (also the pool slice offsets of the serving loop).
+__host__ __device__ inline uint64_t mix64(uint64_t z) {
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
 
 // One complex request (NoTrans / NoTrans, dense column-major interleaved
 // complex f64 operands): op(A) m x k at a, op(B) k x n at b, C m x n at c;

@@ -6,9 +6,13 @@ native serving loop in continuous-batching or plain per-call mode.
 The serving workload: Poisson arrivals at rate `rate` (requests/s), `count`
 requests, 60% gemm (m, n, k log-uniform in [32, 4096], f32 or f64 with
 probability 1/2), 25% trsm (order U[64, 2048], right-hand sides U[1, 512],
-f64), 15% complex f64 gemm (m, n, k log-uniform in [64, 2048]). Operands are
-generated in HBM per request (see csrc/serve.hip); the admission cap is on
-the summed footprint of admitted, unfinished requests (200 GB by default).
+f64), 15% complex f64 gemm (m, n, k log-uniform in [64, 2048]). The A and B
+of gemm and complex requests are slices of a read-only operand pool per
+element type, generated in HBM before the first arrival; the written
+operands (C, a trsm's A and B) are generated in HBM per request (see
+csrc/serve.hip). The admission cap is on the summed footprint (written
+operands and workspace) of admitted, unfinished requests (200 GB by
+default).
 """
 from __future__ import annotations
 
@@ -22,6 +26,9 @@ from .level3 import raise_status
 GEMM, TRSM, ZGEMM = 0, 1, 2
 BATCHED, PLAIN = 0, 1
 HBM_CAP = 200e9
+POOL_SEED = 0x5EEDB10C5EEDB10C  # RBLAS_SERVE_POOL_SEED
+POOL_MIN = 1 << 25              # pool elements per element type, at least
+_M64 = (1 << 64) - 1
 
 REQUEST_DTYPE = np.dtype([("kind", np.int32), ("dtype", np.int32), ("m", np.int32),
                           ("n", np.int32), ("k", np.int32), ("pad", np.int32),
@@ -60,15 +67,17 @@ def workload(count: int = 10_000, rate: float = 0.0, seed: int = 7) -> np.ndarra
 
 
 def run(requests: np.ndarray, mode: int = BATCHED, max_inflight: int = 4, max_batch: int = 32,
-        hbm_cap: float = HBM_CAP, capture: dict | None = None):
+        hbm_cap: float = HBM_CAP, capture: dict | None = None, hash_col_step: int = 1):
     """Serve `requests` (REQUEST_DTYPE). `capture` maps request index -> a
-    host array that receives its dense result. Returns (results, stats)."""
+    host array that receives its dense result; hash_col_step > 1 hashes
+    every hash_col_step-th column of each result only. Returns (results,
+    stats)."""
     requests = np.ascontiguousarray(requests, REQUEST_DTYPE)
     results = np.zeros(len(requests), RESULT_DTYPE)
     capture = capture or {}
     ids = (C.c_int32 * max(1, len(capture)))(*capture.keys())
     outs = (C.c_void_p * max(1, len(capture)))(*[a.ctypes.data for a in capture.values()])
-    cfg = _ffi.ServeConfig(mode, max_inflight, max_batch, len(capture), hbm_cap,
+    cfg = _ffi.ServeConfig(mode, max_inflight, max_batch, len(capture), hash_col_step, 0, hbm_cap,
                            C.cast(ids, C.POINTER(C.c_int32)), C.cast(outs, C.POINTER(C.c_void_p)))
     stats = _ffi.ServeStats()
     st = _ffi.lib().rblas_serve_run(requests.ctypes.data, len(requests), C.byref(cfg),
@@ -88,16 +97,45 @@ def summary(results: np.ndarray, stats: dict) -> dict:
 
 
 # ------------------------------------------------------- host regeneration
-def operands(req) -> tuple:
+def _mix64(z: int) -> int:
+    z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & _M64
+    z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & _M64
+    return z ^ (z >> 31)
+
+
+def pool_elems(requests: np.ndarray) -> dict:
+    """Pool size (elements) per element type (0 f32, 1 f64) of a run over
+    `requests`: 2^25, or the largest pooled operand."""
+    need = {0: 0, 1: 0}
+    for r in np.atleast_1d(requests):
+        kind, m, n, k = (int(r[f]) for f in ("kind", "m", "n", "k"))
+        if kind == TRSM:
+            continue
+        c = 2 if kind == ZGEMM else 1
+        dt = 1 if kind == ZGEMM else int(r["dtype"])
+        need[dt] = max(need[dt], c * m * k, c * k * n)
+    return {dt: max(POOL_MIN, v) for dt, v in need.items()}
+
+
+def _pool_slice(seed: int, j: int, length: int, elems: int, dt) -> np.ndarray:
+    off = _mix64(seed ^ j) % (elems - length + 1)
+    return workloads.uniform(POOL_SEED, off, length, dt)
+
+
+def operands(req, pools: dict | None = None) -> tuple:
     """The request's operands exactly as the device generates them (numpy),
     column-major flattened: gemm (A, B, C0); trsm (A, B0); zgemm complex
-    (A, B, C0)."""
+    (A, B, C0). `pools`: pool_elems() of the run's trace (default: the
+    minimum pool, which any trace whose pooled operands stay within 2^25
+    elements uses)."""
     kind, m, n, k, seed = int(req["kind"]), int(req["m"]), int(req["n"]), int(req["k"]), \
         int(req["seed"])
+    pools = pools or {0: POOL_MIN, 1: POOL_MIN}
     if kind == GEMM:
         dt = np.float32 if int(req["dtype"]) == 0 else np.float64
+        el = pools[int(req["dtype"])]
         la, lb, lc = m * k, k * n, m * n
-        return (workloads.uniform(seed, 0, la, dt), workloads.uniform(seed, la, lb, dt),
+        return (_pool_slice(seed, 1, la, el, dt), _pool_slice(seed, 2, lb, el, dt),
                 workloads.uniform(seed, la + lb, lc, dt))
     if kind == TRSM:
         dt = np.float32 if int(req["dtype"]) == 0 else np.float64
@@ -109,8 +147,8 @@ def operands(req) -> tuple:
     la, lb, lc = 2 * m * k, 2 * k * n, 2 * m * n
     def cplx(x):
         return x.view(np.complex128)
-    return (cplx(workloads.uniform(seed, 0, la, np.float64)),
-            cplx(workloads.uniform(seed, la, lb, np.float64)),
+    return (cplx(_pool_slice(seed, 1, la, pools[1], np.float64)),
+            cplx(_pool_slice(seed, 2, lb, pools[1], np.float64)),
             cplx(workloads.uniform(seed, la + lb, lc, np.float64)))
 
 

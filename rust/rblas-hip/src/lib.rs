@@ -190,6 +190,8 @@ pub mod ffi {
         pub max_inflight: i32,
         pub max_batch: i32,
         pub capture_count: i32,
+        pub hash_col_step: i32,
+        pub pad: i32,
         pub hbm_cap_bytes: f64,
         pub capture_ids: *const i32,
         pub capture_out: *const *mut c_void,

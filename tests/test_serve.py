@@ -117,3 +117,20 @@ def test_serve_admission_cap_holds(gpu):
     assert stats2["peak_footprint_bytes"] <= cap
     assert stats2["batches"] > stats["batches"]  # the cap split the work
     assert np.array_equal(res2["hash"], res["hash"])  # same bits under any admission
+
+
+@pytest.mark.gpu
+def test_serve_sampled_hash_same_alone_and_in_mix(gpu):
+    """hash_col_step > 1 fingerprints every hash_col_step-th column: the same
+    bits alone and in the mix, and a different fingerprint from the full one
+    for a result with more than one column."""
+    req = _small_trace(40, 6, rate=3000.0)
+    res, _ = serve.run(req, serve.BATCHED, max_inflight=3, max_batch=8, hash_col_step=3)
+    full, _ = serve.run(req, serve.BATCHED, max_inflight=3, max_batch=8)
+    for i in range(0, 40, 7):
+        one = req[i:i + 1].copy()
+        one["arrival_s"] = 0.0
+        r1, _ = serve.run(one, serve.BATCHED, hash_col_step=3)
+        assert r1["hash"][0] == res["hash"][i], i
+        if int(req["n"][i]) > 1:
+            assert res["hash"][i] != full["hash"][i], i
