@@ -247,9 +247,13 @@ __global__ __launch_bounds__(kInvThreads) void grouped_trsm_inv(const TrsmDesc<T
   invert_block(descs[u.x], u.y, inv);
 }
 
-// One problem passed by value: block blockIdx.x.
+// One problem passed by value: block blockIdx.x. Workgroup 0 also clears
+// the nflags flags of a split solve.
 template <typename T>
-__global__ __launch_bounds__(kInvThreads) void single_trsm_inv(TrsmDesc<T> d, double *inv) {
+__global__ __launch_bounds__(kInvThreads) void single_trsm_inv(TrsmDesc<T> d, double *inv,
+                                                               int *flags, int nflags) {
+  if (blockIdx.x == 0)
+    for (int i = threadIdx.x; i < nflags; i += kInvThreads) flags[i] = 0;
   invert_block(d, blockIdx.x, inv);
 }
 
@@ -718,6 +722,67 @@ __device__ __forceinline__ void apply_x(const TrsmDesc<T> &d, f64x4 (&a)[CB],
   }
 }
 
+// A unit's solve split over two workgroups (single calls whose units would
+// leave most CUs idle). Part 0 holds the row blocks of the first nblk0
+// diagonal blocks (in substitution order) and publishes each -X_b it forms,
+// f64, at xst + b * kNB * W (row-major, W doubles a row), then raises *flag
+// to b + 1. Part 1 holds the rest: for the blocks before its own it waits
+// for *flag and reads -X_b from xst instead of forming it, then solves its
+// own blocks. Every row gets the same f64 operations in the same order as in
+// one workgroup. The hand-off uses agent-scope relaxed atomics for the data
+// and the flag (written through to, and read from, the level coherent
+// across XCDs) and orders them by waiting for the data stores before the
+// workgroup barrier that precedes the flag store: agent-scope release and
+// acquire fences would write back or invalidate the whole L2 of the XCD on
+// every block, and with it the op(A) rows the units share there.
+// part -1: no split.
+struct Split {
+  int part;
+  int nblk0;
+  int *flag;
+  double *xst;
+};
+
+// Bound on the polls of a part-1 workgroup for one block (each after a
+// short sleep), so that every wave ends even if its part 0 never publishes;
+// -X then reads as NaN, which the result carries.
+constexpr int kSplitPolls = 1 << 24;
+
+// -X_b of part 0 for a part-1 workgroup: waits for the flag, then reads the
+// 16 rows of sub-block oi into Xn in the layout the owners write.
+template <int CB>
+__device__ __forceinline__ void fetch_x(const Split &sp, int b, int oi, int lane, double *Xn,
+                                        int kS) {
+  constexpr int W = 16 * CB;
+  bool ok = true;
+  if (lane == 0) {
+    int polls = 0;
+    while (__hip_atomic_load(sp.flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < b + 1) {
+      __builtin_amdgcn_s_sleep(2);
+      if (++polls == kSplitPolls) {
+        ok = false;
+        break;
+      }
+    }
+  }
+  ok = __shfl(ok ? 1 : 0, 0) != 0;
+  const int li = lane & 15, kq = lane >> 4;
+  const int row = 16 * oi + li;
+  double *src = sp.xst + (int64_t)b * kNB * W + (int64_t)row * W;
+  double v[CB][4];
+#pragma unroll
+  for (int cb = 0; cb < CB; ++cb)
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+      v[cb][r] = __hip_atomic_load(src + 16 * cb + kq + 4 * r, __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+#pragma unroll
+  for (int cb = 0; cb < CB; ++cb)
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+      Xn[row * kS + 16 * cb + kq + 4 * r] = ok ? v[cb][r] : __builtin_nan("");
+}
+
 // Units of 16 CB right-hand sides (CB = 1 or 2; orders up to 16 kWaves RB)
 // held in registers instead of LDS, as f64 MFMA accumulators. Row blocks (16
 // rows) are numbered h = 0, 1, ... in substitution order -- h = g for a
@@ -752,7 +817,8 @@ __device__ __forceinline__ void apply_x(const TrsmDesc<T> &d, f64x4 (&a)[CB],
 // Neither barrier waits for the op(A) loads in flight.
 template <typename T, int RB, int CB>
 __device__ __forceinline__ void solve_unit_reg(const TrsmDesc<T> &d, int c0, const double *inv,
-                                               double *Bs, double *Is0) {
+                                               double *Bs, double *Is0,
+                                               const Split sp = Split{-1, 0, nullptr, nullptr}) {
   static_assert(kWaves == 8, "owners of a diagonal block are waves 0-3 or 4-7");
   constexpr int W = 16 * CB;
   constexpr int kS = W + 1;                 // Bb / Xs row stride in doubles
@@ -772,14 +838,22 @@ __device__ __forceinline__ void solve_unit_reg(const TrsmDesc<T> &d, int c0, con
   const bool upper = d.upper, fwd = d.forward;
   const int li = lane & 15, kq = lane >> 4;
   const int nblk = qpad / kNB;
-  // Accumulators of this wave with h >= 4 st: h = wave + kWaves j, j < nrb.
-  auto live_from = [](int st, int w, int n) {
-    return n > w ? (n - w + kWaves - 1) / kWaves - (4 * st > w ? (4 * st - w + kWaves - 1) / kWaves : 0)
-                 : 0;
+  // This workgroup's row blocks h in [hbeg, hend) (all of them unless split),
+  // and the last diagonal block it solves plus one.
+  const int hbeg = sp.part == 1 ? 4 * sp.nblk0 : 0;
+  const int hend = sp.part == 0 ? 4 * sp.nblk0 : nrb;
+  const int send = sp.part == 0 ? sp.nblk0 : nblk;
+  // Accumulators of this wave with h >= 4 st: h = hbeg + wave + kWaves j < hend.
+  auto live_from = [hbeg, hend](int st, int w, int) {
+    const int first = hbeg + w;
+    if (first >= hend) return 0;
+    const int lo = 4 * st > first ? (4 * st - first + kWaves - 1) / kWaves : 0;
+    return (hend - first + kWaves - 1) / kWaves - lo;
   };
   const int nown = live_from(0, wave, nrb);  // this wave's row blocks
-  dma_inverse(inv + d.inv0 + (int64_t)diag_block(fwd, nblk, 0) * kNB * kNB, upper, Is0, wave,
-              lane);
+  if (hbeg == 0)
+    dma_inverse(inv + d.inv0 + (int64_t)diag_block(fwd, nblk, 0) * kNB * kNB, upper, Is0, wave,
+                lane);
   // B, alpha applied to every entry on load (rblas's pre-scale); row blocks
   // past the order are neither loaded nor stored.
   const int64_t rstep = d.right ? d.ldb : 1, cstep = d.right ? 1 : d.ldb;
@@ -789,7 +863,7 @@ __device__ __forceinline__ void solve_unit_reg(const TrsmDesc<T> &d, int c0, con
     T raw[RB][CB][4];
 #pragma unroll
     for (int j = 0; j < RB; ++j) {
-      const int g = row_block(upper, nrb, wave, j);
+      const int g = row_block(upper, nrb, hbeg + wave, j);
       const bool live = j < nown && 16 * g < q;
       const bool rok = 16 * g + li < q;
 #pragma unroll
@@ -803,7 +877,7 @@ __device__ __forceinline__ void solve_unit_reg(const TrsmDesc<T> &d, int c0, con
     }
 #pragma unroll
     for (int j = 0; j < RB; ++j) {
-      const int g = row_block(upper, nrb, wave, j);
+      const int g = row_block(upper, nrb, hbeg + wave, j);
       const bool rok = j < nown && 16 * g + li < q;
 #pragma unroll
       for (int cb = 0; cb < CB; ++cb)
@@ -819,16 +893,20 @@ __device__ __forceinline__ void solve_unit_reg(const TrsmDesc<T> &d, int c0, con
   T pa[kLoads], pb[kLoads];
   double xq[16];
   int done = 0;  // rotations so far: acc[k] holds the wave's row block j = k + done
-  for (int s = -1; s + 1 < nblk; ++s) {  // the last block's step has nothing left to update
+  for (int s = -1; s + 1 < send; ++s) {  // the last block's step has nothing left to update
     const bool upd = s >= 0;
     const int r0 = upd ? diag_block(fwd, nblk, s) * kNB : 0;
     const int nb = upd ? min(kNB, q - r0) : 0;
     const int hi = upper ? r0 : q;
     const int nact = upd ? live_from(s + 1, wave, nrb) : 0;  // accumulators X_s reaches
-    // Owners of D_{s+1}: h = 4s + 4 + i4, this wave's acc[0].
+    // Owners of D_{s+1}: h = 4s + 4 + i4, this wave's acc[0]. A part-1
+    // workgroup reads the X of part 0's blocks instead (waves 0-3, one
+    // sub-block each).
     const bool next = s + 1 < nblk;
-    const int i4 = wave - ((4 * s + 4) % kWaves);
-    const bool owner = next && i4 >= 0 && i4 < 4;
+    const bool own_next = next && 4 * s + 4 >= hbeg;
+    const int i4 = own_next ? wave - ((4 * s + 4 - hbeg) % kWaves) : wave;
+    const bool owner = own_next && i4 >= 0 && i4 < 4;
+    const bool fetcher = next && !own_next && wave < 4;
     const int oi = upper ? 3 - i4 : i4;  // sub-block within D_{s+1}
     const int r0n = next ? diag_block(fwd, nblk, s + 1) * kNB : 0;
     const int nbn = next ? min(kNB, q - r0n) : 0;
@@ -841,8 +919,8 @@ __device__ __forceinline__ void solve_unit_reg(const TrsmDesc<T> &d, int c0, con
     bool more = false;
     if (nact > 0) {
       more = nact > 1;
-      apply_x<T, CB>(d, acc[0], pa, pb, xq, Xs, kS, 16 * row_block(upper, nrb, wave, done),
-                     16 * row_block(upper, nrb, wave, done + 1), more, hi, r0, nb, si, sj, lane);
+      apply_x<T, CB>(d, acc[0], pa, pb, xq, Xs, kS, 16 * row_block(upper, nrb, hbeg + wave, done),
+                     16 * row_block(upper, nrb, hbeg + wave, done + 1), more, hi, r0, nb, si, sj, lane);
     }
     if (owner) {
       const int row = 16 * oi + li;
@@ -857,27 +935,36 @@ __device__ __forceinline__ void solve_unit_reg(const TrsmDesc<T> &d, int c0, con
     else
       barrier_vm<0>();
     // C.
-    if (s + 2 < nblk)
+    if (s + 2 < send && 4 * (s + 2) >= hbeg)
       dma_inverse(inv + d.inv0 + (int64_t)diag_block(fwd, nblk, s + 2) * kNB * kNB, upper,
                   (s & 1) ? Is1 : Is0, wave, lane);
+    double *Xn = Bs + (1 + ((s + 1) & 1)) * kNB * kS;
     if (owner) {
       f64x4 x[CB];
       form_x<CB>(Bb, kS, (s & 1) ? Is0 : Is1, upper, oi, nbn, lane, x);
-      double *Xn = Bs + (1 + ((s + 1) & 1)) * kNB * kS;
       const int row = 16 * oi + li;
+      double *pub = sp.part == 0 ? sp.xst + (int64_t)(s + 1) * kNB * W + (int64_t)row * W : nullptr;
 #pragma unroll
       for (int cb = 0; cb < CB; ++cb) {
         acc[0][cb] = x[cb];
 #pragma unroll
-        for (int r = 0; r < 4; ++r)
-          Xn[row * kS + 16 * cb + kq + 4 * r] = row < nbn ? -x[cb][r] : 0.0;
+        for (int r = 0; r < 4; ++r) {
+          const double v = row < nbn ? -x[cb][r] : 0.0;
+          Xn[row * kS + 16 * cb + kq + 4 * r] = v;
+          if (sp.part == 0)
+            __hip_atomic_store(pub + 16 * cb + kq + 4 * r, v, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+        }
       }
+      // -X_{s+1} stored before this wave reaches the barrier ahead of the flag.
+      if (sp.part == 0) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
     }
+    if (fetcher) fetch_x<CB>(sp, s + 1, oi, lane, Xn, kS);
 #pragma unroll
     for (int k = 1; k < RB; ++k) {
       if (k < nact) {
-        const int rb = 16 * row_block(upper, nrb, wave, done + k);
-        const int rn = 16 * row_block(upper, nrb, wave, done + k + 1);
+        const int rb = 16 * row_block(upper, nrb, hbeg + wave, done + k);
+        const int rn = 16 * row_block(upper, nrb, hbeg + wave, done + k + 1);
         // CB = 1 alternates the two sets; CB = 2 always starts from pa.
         if (CB == 1 && (k & 1))
           apply_x<T, CB>(d, acc[k], pb, pa, xq, Xs, kS, rb, rn, k + 1 < nact, hi, r0, nb, si, sj,
@@ -893,10 +980,10 @@ __device__ __forceinline__ void solve_unit_reg(const TrsmDesc<T> &d, int c0, con
     if (s + 1 < nblk && live_from(s + 2, wave, nrb) > 0) {
       const int r0x = diag_block(fwd, nblk, s + 1) * kNB;
       if constexpr (CB == 1)
-        load_rows_buf(d, 16 * row_block(upper, nrb, wave, first), upper ? r0x : q, r0x,
+        load_rows_buf(d, 16 * row_block(upper, nrb, hbeg + wave, first), upper ? r0x : q, r0x,
                       min(kNB, q - r0x), si, sj, lane, pa);
       else
-        load_half_buf(d, 16 * row_block(upper, nrb, wave, first), upper ? r0x : q, r0x,
+        load_half_buf(d, 16 * row_block(upper, nrb, hbeg + wave, first), upper ? r0x : q, r0x,
                       min(kNB, q - r0x), si, sj, 0, lane, pa);
       inflight = true;
     }
@@ -904,6 +991,8 @@ __device__ __forceinline__ void solve_unit_reg(const TrsmDesc<T> &d, int c0, con
       barrier_vm<kLoads>();  // X_{s+1} visible
     else
       barrier_vm<0>();
+    if (sp.part == 0 && t == 0)  // every owner wave's stores completed before the barrier
+      __hip_atomic_store(sp.flag, s + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     if (owner) {  // acc[0] solved: rotate it to the back
       f64x4 tmp[CB];
 #pragma unroll
@@ -921,7 +1010,7 @@ __device__ __forceinline__ void solve_unit_reg(const TrsmDesc<T> &d, int c0, con
 #pragma unroll
   for (int k = 0; k < RB; ++k) {
     const int j = (k + done) % RB;
-    const int g = row_block(upper, nrb, wave, j);
+    const int g = row_block(upper, nrb, hbeg + wave, j);
     const bool rok = j < nown && 16 * g + li < q;
     if (j < nown && 16 * g < q) {
 #pragma unroll
@@ -938,12 +1027,13 @@ __device__ __forceinline__ void solve_unit_reg(const TrsmDesc<T> &d, int c0, con
 // LDS one (trsm_unit_width picks).
 template <typename T>
 __device__ __forceinline__ void solve_any(const TrsmDesc<T> &d, int c0, const double *inv,
-                                          double *Bs, double *Is) {
+                                          double *Bs, double *Is,
+                                          const Split sp = Split{-1, 0, nullptr, nullptr}) {
   constexpr int RB = kTrsmLdsDoubles / 16 / (16 * kWaves);
   if (d.width == 32)
-    solve_unit_reg<T, kTrsmWideMaxOrder / (16 * kWaves), 2>(d, c0, inv, Bs, Is);
+    solve_unit_reg<T, kTrsmWideMaxOrder / (16 * kWaves), 2>(d, c0, inv, Bs, Is, sp);
   else if (d.width == 16)
-    solve_unit_reg<T, RB, 1>(d, c0, inv, Bs, Is);
+    solve_unit_reg<T, RB, 1>(d, c0, inv, Bs, Is, sp);
   else
     solve_unit<T, 8>(d, c0, inv, Bs, Is);
 }
@@ -1019,12 +1109,23 @@ __global__ __launch_bounds__(kThreads) void grouped_trsm_kernel(const TrsmDesc<T
 }
 
 // One problem passed by value: workgroup u solves right-hand sides
-// u*width .. u*width + width - 1.
+// u*width .. u*width + width - 1. Split (nblk0 > 0): workgroups u and
+// u + units solve the parts 0 and 1 of unit u (Split), with flags[u] and the
+// published X of unit u at xst + u * nblk * kNB * width; part 0's
+// workgroups have the lower ids, so they are dispatched first.
 template <typename T>
-__global__ __launch_bounds__(kThreads) void single_trsm_solve(TrsmDesc<T> d, const double *inv) {
+__global__ __launch_bounds__(kThreads) void single_trsm_solve(TrsmDesc<T> d, const double *inv,
+                                                              int units, int nblk0, int *flags,
+                                                              double *xst) {
   __shared__ __align__(16) double Bs[kLdsDoubles];
   __shared__ __align__(16) double Is[kInvLds];
-  solve_any(d, (int)blockIdx.x * d.width, inv, Bs, Is);
+  const int u = (int)blockIdx.x % units, part = (int)blockIdx.x / units;
+  Split sp{-1, 0, nullptr, nullptr};
+  if (nblk0 > 0) {
+    const int nblk = (d.q + kNB - 1) / kNB;
+    sp = Split{part, nblk0, flags + u, xst + (int64_t)u * nblk * kNB * d.width};
+  }
+  solve_any(d, u * d.width, inv, Bs, Is, sp);
 }
 
 }  // namespace
@@ -1034,9 +1135,17 @@ hipError_t launch_trsm_single(const TrsmDesc<T> &d, double *inv, hipStream_t st)
   if (d.q <= 0 || d.nrhs <= 0) return hipSuccess;
   const int nblk = (d.q + kNB - 1) / kNB;
   const int units = (d.nrhs + d.width - 1) / d.width;
-  hipLaunchKernelGGL((single_trsm_inv<T>), dim3((unsigned)nblk), dim3(kInvThreads), 0, st, d, inv);
-  hipLaunchKernelGGL((single_trsm_solve<T>), dim3((unsigned)units), dim3(kThreads), 0, st, d,
-                     (const double *)inv);
+  const int nblk0 = trsm_split_blocks(d.q, d.nrhs, d.width);
+  int *flags = nullptr;
+  double *xst = nullptr;
+  if (nblk0 > 0) {
+    xst = inv + (int64_t)nblk * kNB * kNB;
+    flags = (int *)(xst + trsm_split_x_doubles(d.q, d.nrhs));
+  }
+  hipLaunchKernelGGL((single_trsm_inv<T>), dim3((unsigned)nblk), dim3(kInvThreads), 0, st, d, inv,
+                     flags, nblk0 > 0 ? units : 0);
+  hipLaunchKernelGGL((single_trsm_solve<T>), dim3((unsigned)(nblk0 > 0 ? 2 * units : units)),
+                     dim3(kThreads), 0, st, d, (const double *)inv, units, nblk0, flags, xst);
   return hipGetLastError();
 }
 
@@ -1044,7 +1153,8 @@ template <typename T>
 hipError_t launch_trsm_invert(const TrsmDesc<T> &d, double *inv, hipStream_t st) {
   if (d.q <= 0) return hipSuccess;
   const int nblk = (d.q + kNB - 1) / kNB;
-  hipLaunchKernelGGL((single_trsm_inv<T>), dim3((unsigned)nblk), dim3(kInvThreads), 0, st, d, inv);
+  hipLaunchKernelGGL((single_trsm_inv<T>), dim3((unsigned)nblk), dim3(kInvThreads), 0, st, d, inv,
+                     (int *)nullptr, 0);
   return hipGetLastError();
 }
 

@@ -492,9 +492,15 @@ struct Solver {
 }  // namespace
 
 size_t trsm_workspace_bytes(int64_t m, int64_t n, bool left) {
-  const int64_t q = left ? m : n;
+  const int64_t q = left ? m : n, nrhs = left ? n : m;
   const int64_t nblk = (q + kNB - 1) / kNB;
-  return (size_t)(nblk * kNB * kNB) * sizeof(double);
+  size_t bytes = (size_t)(nblk * kNB * kNB) * sizeof(double);
+  // A split register-resident solve (orders up to kTrsmWideMaxOrder): its
+  // published X and one flag per unit (at most nrhs / 16 + 1 units).
+  if (q > (kTrsmSplitMinBlocks - 1) * kNB && q <= kTrsmWideMaxOrder)
+    bytes += (size_t)trsm_split_x_doubles(q, nrhs) * sizeof(double) +
+             (size_t)(nrhs / 16 + 1) * sizeof(int);
+  return bytes;
 }
 
 template <typename T>

@@ -301,3 +301,34 @@ def test_grouped_trsm_batching_workload_sample(gpu):
         assert _residual_ratio(0, p["L"], X, 1.0, B0, p["q"], np.float64) <= 1.0, idx
     plan.close()
     s.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("q,nr", [(705, 40), (769, 17), (1000, 300), (1024, 1024)])
+@pytest.mark.parametrize("side,uplo,trans", [(0, 1, 0), (0, 0, 0), (1, 0, 1), (1, 1, 0)])
+def test_trsm_split_single_call_matches_group(gpu, q, nr, side, uplo, trans):
+    """Single calls whose units fill at most half the CUs run each unit as
+    two workgroups (the second half of the diagonal blocks waits for the X of
+    the first): bit for bit the group's one-workgroup solve of the same
+    problem, forward and backward, Left and Right, within the residual
+    bound."""
+    dtype = np.float64
+    rng = np.random.default_rng(q + nr + 8 * side + 4 * uplo + 2 * trans)
+    m, n = (q, nr) if side == 0 else (nr, q)
+    lda, ldb = q + 3, m + 1
+    a, opA = _system(q, uplo, trans, 0, lda, dtype, rng)
+    b = rng.standard_normal((n - 1) * ldb + m).astype(dtype)
+    single = b.copy()
+    level3.trsm(side, uplo, trans, 0, m, n, 0.5, a, lda, single, ldb)
+    s = Stream()
+    da, db = DeviceArray.from_host(a, s), DeviceArray.from_host(b, s)
+    plan = grouped.GroupedTrsm([grouped.TrsmProblem(side, uplo, trans, 0, m, n, 0.5, da, lda, db,
+                                                    ldb)], dtype)
+    plan.run(s)
+    s.synchronize()
+    assert np.array_equal(single, db.download(s))
+    X = view(single, COL, m, n, ldb)
+    B0 = view(b, COL, m, n, ldb)
+    assert _residual_ratio(side, opA, X, 0.5, B0, q, dtype) <= 1.0
+    plan.close()
+    s.close()
