@@ -41,6 +41,8 @@ def test_serve_rejects_malformed_requests():
     big["kind"], big["m"], big["n"], big["k"] = serve.GEMM, 4096, 4096, 4096
     with pytest.raises(ValueError):
         serve.run(big, hbm_cap=1e6)
+    with pytest.raises(ValueError):
+        serve.run(serve.workload(2, 0.0), hash_col_step=-1)
 
 
 def _small_trace(count, seed, rate):
