@@ -1,6 +1,8 @@
 // Device-side descriptors of the grouped (persistent) launches and their
 // host launch functions.
 #pragma once
+#include <algorithm>
+
 #include "kernels.hpp"
 
 namespace rblas {
@@ -94,22 +96,23 @@ inline bool trsm_wide_ok(int64_t q, int64_t lda, int64_t nrhs) {
   return trsm_unit_width<T>(q, lda) == 16 && q <= kTrsmWideMaxOrder && nrhs > 16;
 }
 
-// A single call's register-resident solve is split over two workgroups per
-// unit (first and second half of the diagonal blocks, in substitution
-// order) when its units would fill at most half the CUs and it has at least
-// kTrsmSplitMinBlocks diagonal blocks: the number of diagonal blocks of part
-// 0, or 0 for no split. width 16 or 32 only (the register-resident solve),
-// orders up to kTrsmWideMaxOrder. (q = nrhs = 1024: f64 0.132 -> 0.127 ms,
-// f32 0.127 -> 0.119 ms; each step's chain is bound by the latency of its
-// op(A) row loads, so halving a workgroup's rows shortens it little, and
-// small orders, with few steps, lost to the hand-off.)
+// A single call's register-resident solve is split over several workgroups
+// per unit (consecutive runs of the diagonal blocks, in substitution order)
+// when its units would leave CUs idle and it has at least
+// kTrsmSplitMinBlocks diagonal blocks: the number of parts (1: no split),
+// at most kTrsmSplitMaxParts and as many as the CUs hold. width 16 or 32
+// only (the register-resident solve), orders up to kTrsmWideMaxOrder. Each
+// step's chain is bound by the latency of its op(A) row loads, so fewer rows
+// per workgroup shorten it only a little, and small orders, with few steps,
+// lose to the hand-off.
 constexpr int kTrsmSplitMinBlocks = 12;
-inline int trsm_split_blocks(int64_t q, int64_t nrhs, int width) {
-  if (width != 16 && width != 32) return 0;
-  if (q < (kTrsmSplitMinBlocks - 1) * 64 + 1 || q > kTrsmWideMaxOrder) return 0;
+constexpr int kTrsmSplitMaxParts = 4;
+inline int trsm_split_parts(int64_t q, int64_t nrhs, int width) {
+  if (width != 16 && width != 32) return 1;
+  if (q < (kTrsmSplitMinBlocks - 1) * 64 + 1 || q > kTrsmWideMaxOrder) return 1;
   const int64_t units = (nrhs + width - 1) / width;
-  if (2 * units > kCUs) return 0;
-  return (int)(((q + 63) / 64) / 2);
+  const int64_t p = std::min<int64_t>(kTrsmSplitMaxParts, kCUs / units);
+  return p >= 2 ? (int)p : 1;
 }
 // Doubles of the published X of a split solve (every unit, every block),
 // for any width: at most (nrhs + 31) right-hand sides of the padded order.

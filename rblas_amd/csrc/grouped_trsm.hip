@@ -722,34 +722,36 @@ __device__ __forceinline__ void apply_x(const TrsmDesc<T> &d, f64x4 (&a)[CB],
   }
 }
 
-// A unit's solve split over two workgroups (single calls whose units would
-// leave most CUs idle). Part 0 holds the row blocks of the first nblk0
-// diagonal blocks (in substitution order) and publishes each -X_b it forms,
-// f64, at xst + b * kNB * W (row-major, W doubles a row), then raises *flag
-// to b + 1. Part 1 holds the rest: for the blocks before its own it waits
-// for *flag and reads -X_b from xst instead of forming it, then solves its
-// own blocks. Every row gets the same f64 operations in the same order as in
-// one workgroup. The hand-off uses agent-scope relaxed atomics for the data
+// A unit's solve split over several workgroups (single calls whose units
+// would leave most CUs idle), one per part: part p holds the row blocks of
+// diagonal blocks [bbeg, bend) (in substitution order). For the blocks
+// before its own it waits for *flag and reads -X_b from xst instead of
+// forming it; then it solves its own blocks and, when later parts exist,
+// publishes each -X_b it forms, f64, at xst + b * kNB * W (row-major, W
+// doubles a row), and raises *flag to b + 1. *flag counts the leading blocks
+// published: a part publishes only after it has read every earlier block,
+// so the parts write it in turn. Every row gets the same f64 operations in
+// the same order as in one workgroup. The hand-off uses agent-scope relaxed atomics for the data
 // and the flag (written through to, and read from, the level coherent
 // across XCDs) and orders them by waiting for the data stores before the
 // workgroup barrier that precedes the flag store: agent-scope release and
 // acquire fences would write back or invalidate the whole L2 of the XCD on
 // every block, and with it the op(A) rows the units share there.
-// part -1: no split.
+// flag null: no split.
 struct Split {
-  int part;
-  int nblk0;
+  int bbeg, bend;
   int *flag;
   double *xst;
 };
+constexpr Split kNoSplit{0, 1 << 24, nullptr, nullptr};  // bend: past any order
 
-// Bound on the polls of a part-1 workgroup for one block (each after a
-// short sleep), so that every wave ends even if its part 0 never publishes;
-// -X then reads as NaN, which the result carries.
+// Bound on the polls of a workgroup for one block (each after a short
+// sleep), so that every wave ends even if the block is never published; -X
+// then reads as NaN, which the result carries.
 constexpr int kSplitPolls = 1 << 24;
 
-// -X_b of part 0 for a part-1 workgroup: waits for the flag, then reads the
-// 16 rows of sub-block oi into Xn in the layout the owners write.
+// -X_b of an earlier part: waits for the flag, then reads the 16 rows of
+// sub-block oi into Xn in the layout the owners write.
 template <int CB>
 __device__ __forceinline__ void fetch_x(const Split &sp, int b, int oi, int lane, double *Xn,
                                         int kS) {
@@ -818,7 +820,7 @@ __device__ __forceinline__ void fetch_x(const Split &sp, int b, int oi, int lane
 template <typename T, int RB, int CB>
 __device__ __forceinline__ void solve_unit_reg(const TrsmDesc<T> &d, int c0, const double *inv,
                                                double *Bs, double *Is0,
-                                               const Split sp = Split{-1, 0, nullptr, nullptr}) {
+                                               const Split sp = kNoSplit) {
   static_assert(kWaves == 8, "owners of a diagonal block are waves 0-3 or 4-7");
   constexpr int W = 16 * CB;
   constexpr int kS = W + 1;                 // Bb / Xs row stride in doubles
@@ -840,9 +842,10 @@ __device__ __forceinline__ void solve_unit_reg(const TrsmDesc<T> &d, int c0, con
   const int nblk = qpad / kNB;
   // This workgroup's row blocks h in [hbeg, hend) (all of them unless split),
   // and the last diagonal block it solves plus one.
-  const int hbeg = sp.part == 1 ? 4 * sp.nblk0 : 0;
-  const int hend = sp.part == 0 ? 4 * sp.nblk0 : nrb;
-  const int send = sp.part == 0 ? sp.nblk0 : nblk;
+  const int hbeg = 4 * sp.bbeg;
+  const int hend = min(nrb, 4 * sp.bend);
+  const int send = min(nblk, sp.bend);
+  const bool publish = sp.flag != nullptr && sp.bend < nblk;  // later parts wait for it
   // Accumulators of this wave with h >= 4 st: h = hbeg + wave + kWaves j < hend.
   auto live_from = [hbeg, hend](int st, int w, int) {
     const int first = hbeg + w;
@@ -943,7 +946,7 @@ __device__ __forceinline__ void solve_unit_reg(const TrsmDesc<T> &d, int c0, con
       f64x4 x[CB];
       form_x<CB>(Bb, kS, (s & 1) ? Is0 : Is1, upper, oi, nbn, lane, x);
       const int row = 16 * oi + li;
-      double *pub = sp.part == 0 ? sp.xst + (int64_t)(s + 1) * kNB * W + (int64_t)row * W : nullptr;
+      double *pub = publish ? sp.xst + (int64_t)(s + 1) * kNB * W + (int64_t)row * W : nullptr;
 #pragma unroll
       for (int cb = 0; cb < CB; ++cb) {
         acc[0][cb] = x[cb];
@@ -951,13 +954,13 @@ __device__ __forceinline__ void solve_unit_reg(const TrsmDesc<T> &d, int c0, con
         for (int r = 0; r < 4; ++r) {
           const double v = row < nbn ? -x[cb][r] : 0.0;
           Xn[row * kS + 16 * cb + kq + 4 * r] = v;
-          if (sp.part == 0)
+          if (publish)
             __hip_atomic_store(pub + 16 * cb + kq + 4 * r, v, __ATOMIC_RELAXED,
                                __HIP_MEMORY_SCOPE_AGENT);
         }
       }
       // -X_{s+1} stored before this wave reaches the barrier ahead of the flag.
-      if (sp.part == 0) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+      if (publish) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
     }
     if (fetcher) fetch_x<CB>(sp, s + 1, oi, lane, Xn, kS);
 #pragma unroll
@@ -991,7 +994,7 @@ __device__ __forceinline__ void solve_unit_reg(const TrsmDesc<T> &d, int c0, con
       barrier_vm<kLoads>();  // X_{s+1} visible
     else
       barrier_vm<0>();
-    if (sp.part == 0 && t == 0)  // every owner wave's stores completed before the barrier
+    if (publish && own_next && t == 0)  // the owners formed X_{s+1}; their stores completed
       __hip_atomic_store(sp.flag, s + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     if (owner) {  // acc[0] solved: rotate it to the back
       f64x4 tmp[CB];
@@ -1028,7 +1031,7 @@ __device__ __forceinline__ void solve_unit_reg(const TrsmDesc<T> &d, int c0, con
 template <typename T>
 __device__ __forceinline__ void solve_any(const TrsmDesc<T> &d, int c0, const double *inv,
                                           double *Bs, double *Is,
-                                          const Split sp = Split{-1, 0, nullptr, nullptr}) {
+                                          const Split sp = kNoSplit) {
   constexpr int RB = kTrsmLdsDoubles / 16 / (16 * kWaves);
   if (d.width == 32)
     solve_unit_reg<T, kTrsmWideMaxOrder / (16 * kWaves), 2>(d, c0, inv, Bs, Is, sp);
@@ -1109,21 +1112,23 @@ __global__ __launch_bounds__(kThreads) void grouped_trsm_kernel(const TrsmDesc<T
 }
 
 // One problem passed by value: workgroup u solves right-hand sides
-// u*width .. u*width + width - 1. Split (nblk0 > 0): workgroups u and
-// u + units solve the parts 0 and 1 of unit u (Split), with flags[u] and the
-// published X of unit u at xst + u * nblk * kNB * width; part 0's
-// workgroups have the lower ids, so they are dispatched first.
+// u*width .. u*width + width - 1. Split (parts > 1): workgroup u + p units
+// solves part p of unit u (diagonal blocks [p nblk / parts, (p + 1) nblk /
+// parts)), with flags[u] and the published X of unit u at
+// xst + u * nblk * kNB * width; earlier parts have the lower ids, so they are
+// dispatched first.
 template <typename T>
 __global__ __launch_bounds__(kThreads) void single_trsm_solve(TrsmDesc<T> d, const double *inv,
-                                                              int units, int nblk0, int *flags,
+                                                              int units, int parts, int *flags,
                                                               double *xst) {
   __shared__ __align__(16) double Bs[kLdsDoubles];
   __shared__ __align__(16) double Is[kInvLds];
   const int u = (int)blockIdx.x % units, part = (int)blockIdx.x / units;
-  Split sp{-1, 0, nullptr, nullptr};
-  if (nblk0 > 0) {
+  Split sp = kNoSplit;
+  if (parts > 1) {
     const int nblk = (d.q + kNB - 1) / kNB;
-    sp = Split{part, nblk0, flags + u, xst + (int64_t)u * nblk * kNB * d.width};
+    sp = Split{part * nblk / parts, (part + 1) * nblk / parts, flags + u,
+               xst + (int64_t)u * nblk * kNB * d.width};
   }
   solve_any(d, u * d.width, inv, Bs, Is, sp);
 }
@@ -1135,17 +1140,17 @@ hipError_t launch_trsm_single(const TrsmDesc<T> &d, double *inv, hipStream_t st)
   if (d.q <= 0 || d.nrhs <= 0) return hipSuccess;
   const int nblk = (d.q + kNB - 1) / kNB;
   const int units = (d.nrhs + d.width - 1) / d.width;
-  const int nblk0 = trsm_split_blocks(d.q, d.nrhs, d.width);
+  const int parts = trsm_split_parts(d.q, d.nrhs, d.width);
   int *flags = nullptr;
   double *xst = nullptr;
-  if (nblk0 > 0) {
+  if (parts > 1) {
     xst = inv + (int64_t)nblk * kNB * kNB;
     flags = (int *)(xst + trsm_split_x_doubles(d.q, d.nrhs));
   }
   hipLaunchKernelGGL((single_trsm_inv<T>), dim3((unsigned)nblk), dim3(kInvThreads), 0, st, d, inv,
-                     flags, nblk0 > 0 ? units : 0);
-  hipLaunchKernelGGL((single_trsm_solve<T>), dim3((unsigned)(nblk0 > 0 ? 2 * units : units)),
-                     dim3(kThreads), 0, st, d, (const double *)inv, units, nblk0, flags, xst);
+                     flags, parts > 1 ? units : 0);
+  hipLaunchKernelGGL((single_trsm_solve<T>), dim3((unsigned)(parts * units)), dim3(kThreads), 0,
+                     st, d, (const double *)inv, units, parts, flags, xst);
   return hipGetLastError();
 }
 
