@@ -731,12 +731,16 @@ __device__ __forceinline__ void apply_x(const TrsmDesc<T> &d, f64x4 (&a)[CB],
 // doubles a row), and raises *flag to b + 1. *flag counts the leading blocks
 // published: a part publishes only after it has read every earlier block,
 // so the parts write it in turn. Every row gets the same f64 operations in
-// the same order as in one workgroup. The hand-off uses agent-scope relaxed atomics for the data
-// and the flag (written through to, and read from, the level coherent
-// across XCDs) and orders them by waiting for the data stores before the
-// workgroup barrier that precedes the flag store: agent-scope release and
-// acquire fences would write back or invalidate the whole L2 of the XCD on
-// every block, and with it the op(A) rows the units share there.
+// the same order as in one workgroup.
+// Data and flag go through relaxed agent-scope atomics, which are performed
+// at the level coherent across XCDs (stores written through to it, loads
+// served from it): the owner waves wait for their data stores to complete
+// (a workgroup-scope release) before the barrier after which thread 0
+// stores the flag, and the reader issues its data loads only after its poll
+// has returned the new flag. Agent-scope release / acquire fences instead
+// would write back or invalidate the XCD's whole L2 on every block (and
+// with it the op(A) rows the units share there): with a release on the flag
+// store alone, q = nrhs = 1024 took 0.119 ms against 0.112.
 // flag null: no split.
 struct Split {
   int bbeg, bend;
@@ -902,9 +906,9 @@ __device__ __forceinline__ void solve_unit_reg(const TrsmDesc<T> &d, int c0, con
     const int nb = upd ? min(kNB, q - r0) : 0;
     const int hi = upper ? r0 : q;
     const int nact = upd ? live_from(s + 1, wave, nrb) : 0;  // accumulators X_s reaches
-    // Owners of D_{s+1}: h = 4s + 4 + i4, this wave's acc[0]. A part-1
-    // workgroup reads the X of part 0's blocks instead (waves 0-3, one
-    // sub-block each).
+    // Owners of D_{s+1}: h = 4s + 4 + i4, this wave's acc[0]. For a block of
+    // an earlier part, waves 0-3 read its published X instead (one sub-block
+    // each).
     const bool next = s + 1 < nblk;
     const bool own_next = next && 4 * s + 4 >= hbeg;
     const int i4 = own_next ? wave - ((4 * s + 4 - hbeg) % kWaves) : wave;
