@@ -110,10 +110,18 @@ def copy_calibrate():
     print(json.dumps({"copy_calibration_bytes": n, "copies": 5}), flush=True)
 
 
-def timed(fn, stream, steps, warmup):
+def timed(fn, stream, steps, warmup, min_warm_s=0.05):
+    """ms per call of fn over `steps` calls, after at least `warmup` calls and
+    at least min_warm_s seconds of them (a few short launches leave the clocks
+    below their steady state: the first grouped run measured 7% slow)."""
     from rblas_amd.device import Event
-    for _ in range(warmup):
+    t0 = time.perf_counter()
+    done = 0
+    while done < warmup or time.perf_counter() - t0 < min_warm_s:
         fn()
+        done += 1
+        if done >= warmup:
+            stream.synchronize()
     stream.synchronize()
     e0, e1 = Event(), Event()
     e0.record(stream)
@@ -127,9 +135,10 @@ def timed(fn, stream, steps, warmup):
     return ms
 
 
-def square_gemm(dtype, n, ld, steps, warmup, compute=None, operands=None):
+def square_gemm(dtype, n, ld, steps, warmup, compute=None, operands=None, min_warm_s=0.05):
     """ms per call of the 8192-style gemm (optionally a split-SGEMM compute
-    mode); returns (ms, device C downloaded or None)."""
+    mode). min_warm_s: see timed (the headline line passes 0: exactly
+    --warmup calls)."""
     from rblas_amd import level3, workloads
     from rblas_amd.device import DeviceArray, Stream
     la = (n - 1) * ld + n
@@ -143,7 +152,7 @@ def square_gemm(dtype, n, ld, steps, warmup, compute=None, operands=None):
     else:
         fn = lambda: level3.sgemm_ex_dev(1, 0, 0, n, n, n, 1.5, da, ld, db, ld, -0.25, dc, ld,
                                          compute, s)
-    ms = timed(fn, s, steps, warmup)
+    ms = timed(fn, s, steps, warmup, min_warm_s)
     for x in (da, db, dc):
         x.free()
     s.close()
@@ -771,7 +780,7 @@ def main():
     ld = n + 7
     flop = 2.0 * n * n * n
     log("dgemm ...")
-    ms = square_gemm(np.float64, n, ld, args.steps, args.warmup)
+    ms = square_gemm(np.float64, n, ld, args.steps, args.warmup, min_warm_s=0.0)
     tflops = flop / (ms * 1e-3) / 1e12
     out = {
         "metric": METRIC,

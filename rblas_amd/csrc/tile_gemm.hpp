@@ -382,30 +382,23 @@ struct TileGemm {
       stage_barrier<0>();
   }
 
-  // Fragment groups [T0, T1) of the k-tile in stage S. EDGE: only blocks
-  // i < nbi, j < nbj are loaded and multiplied (wave-uniform branches); a
-  // skipped block's accumulators stay zero and reach no stored C entry.
-  template <int S, bool EDGE, int T0 = 0, int T1 = G::NT>
+  // Fragment groups [T0, T1) of the k-tile in stage S for the blocks
+  // i < NI, j < NJ (the blocks of this wave that reach into the matrix;
+  // NI == FM and NJ == FN on an interior tile). Compile-time counts keep the
+  // code straight-line; a skipped block's accumulators stay zero and reach
+  // no stored C entry.
+  template <int S, int NI, int NJ, int T0 = 0, int T1 = G::NT>
   __device__ __forceinline__ void compute() {
     const char *As = lds_a<S>();
     const char *Bs = lds_b<S>();
     const int g = lane >> 4, c = lane & 15;
     typedef T vec_t __attribute__((ext_vector_type(G::V)));
-    // Opaque copies: the block tests are made here, per step, rather than
-    // hoisted out of the k loop as lane masks that would stay live in SGPRs.
-    int nbi = __builtin_amdgcn_readfirstlane(this->nbi);
-    int nbj = __builtin_amdgcn_readfirstlane(this->nbj);
-    if (EDGE) {
-      asm volatile("" : "+s"(nbi));
-      asm volatile("" : "+s"(nbj));
-    }
 #pragma unroll
     for (int t = T0; t < T1; ++t) {
       T fa[G::V][FM], fb[G::V][FN];
       const int kb = 4 * G::V * t + G::V * g;  // this lane group's first k
 #pragma unroll
-      for (int i = 0; i < FM; ++i) {
-        if (EDGE && i >= nbi) continue;
+      for (int i = 0; i < NI; ++i) {
         const int mn = row_of(i);
         if constexpr (A_MN) {
 #pragma unroll
@@ -418,8 +411,7 @@ struct TileGemm {
         }
       }
 #pragma unroll
-      for (int j = 0; j < FN; ++j) {
-        if (EDGE && j >= nbj) continue;
+      for (int j = 0; j < NJ; ++j) {
         const int mn = 16 * (4 * j + wsimd) + c;
         if constexpr (B_MN) {
 #pragma unroll
@@ -434,11 +426,9 @@ struct TileGemm {
 #pragma unroll
       for (int p = 0; p < G::V; ++p)
 #pragma unroll
-        for (int i = 0; i < FM; ++i)
+        for (int i = 0; i < NI; ++i)
 #pragma unroll
-          for (int j = 0; j < FN; ++j)
-            if (!EDGE || (i < nbi && j < nbj))
-              acc[i][j] = Mfma<T>::mma(fb[p][j], fa[p][i], acc[i][j]);
+          for (int j = 0; j < NJ; ++j) acc[i][j] = Mfma<T>::mma(fb[p][j], fa[p][i], acc[i][j]);
     }
   }
 
@@ -447,25 +437,48 @@ struct TileGemm {
   // address work overlaps MFMAs already in flight (+1.4% mean over the eight
   // SGEMM 8192 layouts); f64 issues it right after the barrier, which
   // measured faster there. The k order, and so every result bit, is the same.
-  template <int S, bool EDGE>
+  template <int S, int NI, int NJ>
   __device__ __forceinline__ void step(int kt) {
     if (kt >= ktiles) return;
     sync_tile(kt);  // tile kt visible to all waves; the stage of kt-1 is free
     constexpr int N = (S + kStages - 1) % kStages;
     constexpr int H = sizeof(T) == 4 ? G::NT / 2 : 0;
-    compute<S, EDGE, 0, H>();
+    compute<S, NI, NJ, 0, H>();
     issue(kt + kStages - 1, lds_a<N>(), lds_b<N>());
-    compute<S, EDGE, H, G::NT>();
+    compute<S, NI, NJ, H, G::NT>();
   }
 
-  template <bool EDGE>
+  template <int NI, int NJ>
   __device__ __forceinline__ void kloop() {
     for (int kt = 0; kt < ktiles; kt += kStages) {
-      step<0, EDGE>(kt);
-      step<1, EDGE>(kt + 1);
-      step<2, EDGE>(kt + 2);
-      step<3, EDGE>(kt + 3);
+      step<0, NI, NJ>(kt);
+      step<1, NI, NJ>(kt + 1);
+      step<2, NI, NJ>(kt + 2);
+      step<3, NI, NJ>(kt + 3);
     }
+  }
+
+  // The k loop for this wave's block counts nbi in [1, I], nbj == J.
+  template <int I, int J>
+  __device__ __forceinline__ void kloop_rows() {
+    if constexpr (I > 1) {
+      if (nbi < I) {
+        kloop_rows<I - 1, J>();
+        return;
+      }
+    }
+    kloop<I, J>();
+  }
+  // ... for nbi in [1, FM], nbj in [1, J].
+  template <int J>
+  __device__ __forceinline__ void kloop_cols() {
+    if constexpr (J > 1) {
+      if (nbj < J) {
+        kloop_cols<J - 1>();
+        return;
+      }
+    }
+    kloop_rows<FM, J>();
   }
 
   // The first 3 stages of the pipeline.
@@ -475,15 +488,19 @@ struct TileGemm {
     issue(2, lds_a<2>(), lds_b<2>());
   }
 
-  // The k loop after the prologue, unrolled by the 4 stages. A wave whose
-  // blocks all lie inside the matrix runs the branch-free loop; on an edge
-  // tile the others run the one that skips blocks past the edge (same
-  // barriers and DMA, so the waves stay in step).
+  // The k loop after the prologue, unrolled by the 4 stages, in a copy for
+  // each count of this wave's row and column blocks that reach into the
+  // matrix (wave-uniform): all of them on an interior tile, fewer on an edge
+  // tile, none for a wave wholly past the edge (it only stages and waits).
+  // Every copy has the same barriers and DMA, so the waves stay in step, and
+  // each is straight-line, so LDS reads overlap MFMAs as on a full tile.
   __device__ __forceinline__ void kloops() {
     if (nbi == FM && nbj == FN)
-      kloop<false>();
+      kloop<FM, FN>();
+    else if (nbi == 0 || nbj == 0)
+      kloop<0, 0>();
     else
-      kloop<true>();
+      kloop_cols<FN>();
   }
 
   __device__ __forceinline__ void mainloop() {
