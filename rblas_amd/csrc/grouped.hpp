@@ -85,6 +85,10 @@ static_assert((kTrsmQueueStart + kTrsmQueues + 1) * 4 <= kTrsmSchedBytes, "sched
 // only while the units still fill about three quarters of the CUs
 // (kTrsmFillUnits).
 constexpr int kTrsmWideMaxOrder = 1024;
+// A unit {problem, first right-hand side} of a grouped trsm stores its
+// problem as ~problem (negative) when it runs 16 wide in a 32-wide problem
+// (the problem's last unit, at most 16 right-hand sides).
+__host__ __device__ inline int trsm_unit_problem(int x) { return x < 0 ? ~x : x; }
 constexpr int kTrsmFillUnits = 3 * kCUs / 4;
 template <typename T>
 inline int trsm_unit_width(int64_t q, int64_t lda) {

@@ -228,28 +228,37 @@ void TrsmGroup<T>::finish() {
     total = after;
   }
 
-  // Units, each problem's consecutively.
+  // Units, each problem's consecutively. A 32-wide problem whose last unit
+  // holds at most 16 right-hand sides runs that unit 16 wide (its problem
+  // index stored as ~p, see trsm_unit_problem): half the MFMA work for the
+  // same result.
   units.clear();
   cost.clear();
   for (size_t p = 0; p < descs.size(); ++p) {
     const TrsmDesc<T> &d = descs[p];
-    const double unit_us = trsm_unit_us(d.q, d.width);
     for (int c0 = 0; c0 < d.nrhs; c0 += d.width) {
-      units.push_back(int2{(int)p, c0});
-      cost.push_back(unit_us);
+      const bool narrow = d.width == 32 && d.nrhs - c0 <= 16;
+      units.push_back(int2{narrow ? ~(int)p : (int)p, c0});
+      cost.push_back(trsm_unit_us(d.q, narrow ? 16 : d.width));
     }
   }
-  // Each problem's units are consecutive, all of one cost.
+  // Each problem's units are consecutive; a chunk is priced by its first
+  // (widest) unit when ordered and by all of its units when dealt.
   struct Chunk {
     size_t u0, n;
-    double unit_cost;
+    double unit_cost, total;
   };
   constexpr size_t kChunk = kCUs / kTrsmQueues;
   std::vector<Chunk> chunks;
   for (size_t u = 0; u < units.size();) {
     size_t e = u;
-    while (e < units.size() && units[e].x == units[u].x) ++e;
-    for (size_t c = u; c < e; c += kChunk) chunks.push_back(Chunk{c, std::min(kChunk, e - c), cost[u]});
+    while (e < units.size() && trsm_unit_problem(units[e].x) == trsm_unit_problem(units[u].x)) ++e;
+    for (size_t c = u; c < e; c += kChunk) {
+      const size_t n = std::min(kChunk, e - c);
+      double total = 0;
+      for (size_t i = c; i < c + n; ++i) total += cost[i];
+      chunks.push_back(Chunk{c, n, cost[c], total});
+    }
     u = e;
   }
   std::stable_sort(chunks.begin(), chunks.end(),
@@ -260,7 +269,7 @@ void TrsmGroup<T>::finish() {
     int best = 0;
     for (int x = 1; x < kTrsmQueues; ++x)
       if (load[x] < load[best]) best = x;
-    load[best] += c.unit_cost * (double)c.n;
+    load[best] += c.total;
     q[best].insert(q[best].end(), units.begin() + c.u0, units.begin() + c.u0 + c.n);
   }
   std::vector<int2> sorted;

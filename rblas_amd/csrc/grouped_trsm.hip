@@ -1104,7 +1104,9 @@ __global__ __launch_bounds__(kThreads) void grouped_trsm_kernel(const TrsmDesc<T
     __syncthreads();
     if (u < 0) break;
     const int2 w = units[u];
-    const TrsmDesc<T> d = descs[uni(w.x)];
+    const int x = uni(w.x);
+    TrsmDesc<T> d = descs[trsm_unit_problem(x)];
+    if (x < 0) d.width = 16;
     solve_any(d, uni(w.y), inv, Bs, Is);
     __syncthreads();
   }
