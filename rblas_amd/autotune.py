@@ -23,6 +23,7 @@ from __future__ import annotations
 import json
 import os
 import sys
+import time
 
 import numpy as np
 
@@ -34,9 +35,15 @@ TABLE = os.path.join(HERE, "tuning_gfx950.json")
 BIG = 1e9
 
 
-def _time(fn, stream, reps):
+def _time(fn, stream, reps, min_warm_s=0.05):
+    """ms per call over `reps` calls, after warm-up calls filling at least
+    min_warm_s seconds (short launches alone leave the clocks low)."""
+    t0 = time.perf_counter()
     fn()
     stream.synchronize()
+    while time.perf_counter() - t0 < min_warm_s:
+        fn()
+        stream.synchronize()
     e0, e1 = Event(), Event()
     e0.record(stream)
     for _ in range(reps):
