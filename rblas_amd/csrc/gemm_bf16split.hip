@@ -356,35 +356,49 @@ __global__ __launch_bounds__(Waves<W>::kThreads) void split_gemm_kernel(SplitArg
 #pragma unroll
       for (int j = 0; j < kFN; ++j)
 #pragma unroll
-        for (int r = 0; r < 16; ++r) mine[((i * kFN + j) * 16 + r) * kThreads] = g.acc[i][j][r];
-    __threadfence();
+        for (int r = 0; r < 16; ++r)
+          __hip_atomic_store(mine + ((i * kFN + j) * 16 + r) * kThreads, g.acc[i][j][r],
+                             __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    // The partial goes through agent-scope atomic stores (written through to
+    // the level coherent across XCDs) and this wave waits for them before
+    // the barrier after which the split counts itself; the last split reads
+    // the partials with agent-scope atomic loads. No L2 writeback or
+    // invalidation (what agent-scope fences would cost every tile sharing
+    // the XCD's L2).
+    asm volatile("" ::: "memory");
+    __builtin_amdgcn_s_waitcnt(0x0070);  // vmcnt(0) lgkmcnt(0)
+    asm volatile("" ::: "memory");
     __syncthreads();
-    if (threadIdx.x == 0) s_last = atomicAdd(&p.counters[tile], 1) == p.splits - 1;
+    if (threadIdx.x == 0)
+      s_last = __hip_atomic_fetch_add(&p.counters[tile], 1, __ATOMIC_RELAXED,
+                                      __HIP_MEMORY_SCOPE_AGENT) == p.splits - 1;
     __syncthreads();
     if (!s_last) return;
-    __threadfence();
     // acc := slot 0 + slot 1 + ... in split order (this split's own slot is
     // read back too, so the sum is the same whichever split finishes last).
     // One block row (32 values) per pass, its loads issued together: a whole
     // slot at once (128 values beside the 128 accumulators) would crowd the
     // registers.
-    const float *base = p.partials + (int64_t)tile * p.splits * kSlot + threadIdx.x;
+    float *base = p.partials + (int64_t)tile * p.splits * kSlot + threadIdx.x;
 #pragma unroll
     for (int i = 0; i < 4; ++i)
       for (int q = 0; q < p.splits; ++q) {
-        const float *slot = base + (int64_t)q * kSlot;
+        float *slot = base + (int64_t)q * kSlot;
         float v[kFN][16];
 #pragma unroll
         for (int j = 0; j < kFN; ++j)
 #pragma unroll
-          for (int r = 0; r < 16; ++r) v[j][r] = slot[((i * kFN + j) * 16 + r) * kThreads];
+          for (int r = 0; r < 16; ++r)
+            v[j][r] = __hip_atomic_load(slot + ((i * kFN + j) * 16 + r) * kThreads,
+                                        __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 #pragma unroll
         for (int j = 0; j < kFN; ++j)
 #pragma unroll
           for (int r = 0; r < 16; ++r)
             g.acc[i][j][r] = q == 0 ? v[j][r] : g.acc[i][j][r] + v[j][r];
       }
-    if (threadIdx.x == 0) atomicExch(&p.counters[tile], 0);
+    if (threadIdx.x == 0)
+      __hip_atomic_store(&p.counters[tile], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
 
   // Element r of lane l in block (i, j): C row m0+wm+32i+l%32, column

@@ -580,7 +580,9 @@ struct TileGemm {
 
   // Split-k: store this split's raw accumulators in slot `slot` of `ws`
   // (kAccPerThread * kThreads values per slot, in register order, so the
-  // same thread of any workgroup running this tile reads them back).
+  // same thread of any workgroup running this tile reads them back). Relaxed
+  // agent-scope atomic stores: written through to the level coherent across
+  // XCDs, so no L2 writeback is needed to hand them over.
   __device__ __forceinline__ void store_partial(T *ws, int64_t slot) const {
     T *base = ws + slot * (int64_t)(kAccPerThread * kThreads) + tid;
 #pragma unroll
@@ -588,51 +590,61 @@ struct TileGemm {
 #pragma unroll
       for (int j = 0; j < FN; ++j)
 #pragma unroll
-        for (int r = 0; r < 4; ++r) base[((i * FN + j) * 4 + r) * kThreads] = acc[i][j][r];
+        for (int r = 0; r < 4; ++r)
+          __hip_atomic_store(base + ((i * FN + j) * 4 + r) * kThreads, acc[i][j][r],
+                             __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
 
   // acc := slot slot0 + slot slot0+1 + ... + slot slot0+splits-1, added in
   // that order (this split's own slot is read back too): a fixed order, so
   // the result does not depend on which workgroup finishes last. Splits are
-  // the outer loop, so each pass issues all of its loads at once.
-  __device__ __forceinline__ void reduce_partials(const T *ws, int64_t slot0, int splits) {
+  // the outer loop, so each pass issues all of its loads at once. Relaxed
+  // agent-scope atomic loads, served from the level the stores reached.
+  __device__ __forceinline__ void reduce_partials(T *ws, int64_t slot0, int splits) {
     const int64_t stride = (int64_t)(kAccPerThread * kThreads);
     for (int s = 0; s < splits; ++s) {
-      const T *slot = ws + (slot0 + s) * stride + tid;
+      T *slot = ws + (slot0 + s) * stride + tid;
 #pragma unroll
       for (int i = 0; i < FM; ++i)
 #pragma unroll
         for (int j = 0; j < FN; ++j)
 #pragma unroll
           for (int r = 0; r < 4; ++r) {
-            const T v = slot[((i * FN + j) * 4 + r) * kThreads];
+            const T v = __hip_atomic_load(slot + ((i * FN + j) * 4 + r) * kThreads,
+                                          __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             acc[i][j][r] = s == 0 ? v : acc[i][j][r] + v;
           }
     }
   }
 };
 
-// Split-k hand-off for one tile: every split stores its partial, publishes
-// it (device-scope release) and counts itself in `counter`; the split that
-// counts last (acquire) reduces all partials in split order and applies the
-// epilogue, then resets the counter to 0 for the next use. Returns true in
-// the workgroup that must run the epilogue.
+// Split-k hand-off for one tile: every split stores its partial (agent-scope
+// atomic stores), waits until those stores have completed, and once every
+// wave has (barrier) counts itself in `counter` (agent-scope atomic); the
+// split that counts last reduces all partials in split order (agent-scope
+// atomic loads, issued after its count returned) and applies the epilogue,
+// then resets the counter to 0 for the next use. Returns true in the
+// workgroup that must run the epilogue. An agent-scope release / acquire
+// fence pair would also order this, but writes back and then invalidates the
+// XCD's whole L2 (the operand panels the other tiles are reading from it).
 template <typename TG, typename T>
 __device__ __forceinline__ bool split_handoff(TG &tg, T *ws, int64_t slot0, int splits, int mine,
                                               int *counter) {
   __shared__ int s_last;
   tg.store_partial(ws, slot0 + mine);
-  __threadfence();
+  asm volatile("" ::: "memory");
+  __builtin_amdgcn_s_waitcnt(0x0070);  // vmcnt(0) lgkmcnt(0): this wave's partial is stored
+  asm volatile("" ::: "memory");
   __syncthreads();
   if (threadIdx.x == 0) {
-    const int prev = atomicAdd(counter, 1);
+    const int prev = __hip_atomic_fetch_add(counter, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     s_last = prev == splits - 1;
   }
   __syncthreads();
   if (!s_last) return false;
-  __threadfence();
   tg.reduce_partials(ws, slot0, splits);
-  if (threadIdx.x == 0) atomicExch(counter, 0);
+  if (threadIdx.x == 0)
+    __hip_atomic_store(counter, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   return true;
 }
 
