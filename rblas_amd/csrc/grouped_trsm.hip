@@ -29,7 +29,11 @@ namespace {
 #endif
 constexpr int kWaves = RBLAS_TRSM_WAVES;  // solve workgroup: 8 waves
 constexpr int kThreads = 64 * kWaves;
-constexpr int kInvThreads = 1024;   // inversion workgroup: 16 waves, 4 columns each
+// Inversion workgroups: 8 waves for the batched launch (four share a CU),
+// 16 for a single call's few blocks (latency-bound: more lanes to stage and
+// store each block).
+constexpr int kInvThreads = 512;
+constexpr int kInvThreadsSingle = 1024;
 constexpr int kNB = 64;             // diagonal block
 constexpr int kLdsDoubles = kTrsmLdsDoubles;  // q * W limit (128 KiB)
 constexpr int kSub = 256;           // doubles in a 16 x 16 sub-block
@@ -92,18 +96,53 @@ __device__ __forceinline__ double row_bcast(double v, int s) {
   }
 }
 
-typedef double LdsMat[kNB][kNB + 1];  // M[col][row], 64 x 64
+// LDS of one diagonal-block inversion: the lower triangle of a 64 x 64 block
+// as its 16 x 16 sub-blocks (I, J), I >= J, 256 doubles each, element (r, c)
+// of a sub-block at c * 16 + (r ^ c): the 16 rows of a column are contiguous
+// (a permutation of them), and the 16 columns of one row fall on distinct
+// banks. Ls holds the six strictly lower sub-blocks of L; Iv the ten of the
+// inverse. The four diagonal sub-blocks of L live in Iv's slots 1, 3, 4 and 6
+// (off-diagonal slots of the inverse, written only after the diagonal
+// inverses have been formed from them). 32 KiB in all, so four inversions
+// (8 waves each) share a CU. (With 4 waves, five fit the 160 KiB and the
+// batching workload's 1256 blocks run in one round, but each block took
+// longer and the launch no less: 25.6 us.)
+constexpr int kInvOff = 6;   // strictly lower 16 x 16 sub-blocks
+constexpr int kInvTri = 10;  // lower sub-blocks including the diagonal
+__device__ __forceinline__ int sub_elem(int r, int c) { return c * 16 + (r ^ c); }
+// Slot of sub-block (I, J), I > J, in Ls; of (I, J), I >= J, in Iv; of the
+// diagonal sub-block I of L (in Iv).
+__device__ __forceinline__ int ls_slot(int I, int J) { return I * (I - 1) / 2 + J; }
+__device__ __forceinline__ int iv_slot(int I, int J) { return I * (I + 1) / 2 + J; }
+__device__ __forceinline__ int ld_slot(int I) { return I == 0 ? 1 : I == 1 ? 3 : I == 2 ? 4 : 6; }
 
-// acc += X[rx.., cx..] * Y[ry.., cy..] for 16 x 16 blocks of LDS matrices
-// (f64 MFMA, k in steps of 4). Output layout: acc[r] holds row
+struct InvLds {
+  double ls[kInvOff * kSub];
+  double iv[kInvTri * kSub];
+  // L(row, col) of the padded block, lower form (0 above the diagonal).
+  __device__ __forceinline__ double l(int row, int col) const {
+    const int I = row >> 4, J = col >> 4;
+    if (I == J) return iv[ld_slot(I) * kSub + sub_elem(row & 15, col & 15)];
+    if (I > J) return ls[ls_slot(I, J) * kSub + sub_elem(row & 15, col & 15)];
+    return 0.0;
+  }
+  // Element (row, col) of the inverse, row >> 4 >= col >> 4.
+  __device__ __forceinline__ double &inv(int row, int col) {
+    return iv[iv_slot(row >> 4, col >> 4) * kSub + sub_elem(row & 15, col & 15)];
+  }
+};
+
+// acc += L[bi, bk] * Inv[bk, bj] over one 16 x 16 block product (f64 MFMA,
+// k in steps of 4), bi > bk >= bj. Output layout: acc[r] holds row
 // (lane >> 4) + 4 r, column lane & 15 -- also the B-operand layout of the
 // next product's k step r.
-__device__ __forceinline__ f64x4 mm16(const LdsMat &X, int rx, int cx, const LdsMat &Y, int ry,
-                                      int cy, f64x4 acc, int lane) {
+__device__ __forceinline__ f64x4 mm16(InvLds &S, int bi, int bk, int bj, f64x4 acc, int lane) {
+  const double *X = S.ls + ls_slot(bi, bk) * kSub;
+  const double *Y = S.iv + iv_slot(bk, bj) * kSub;
 #pragma unroll
   for (int t4 = 0; t4 < 4; ++t4) {
     const int m = 4 * t4 + (lane >> 4);
-    acc = mma64(X[cx + m][rx + (lane & 15)], Y[cy + (lane & 15)][ry + m], acc);
+    acc = mma64(X[sub_elem(lane & 15, m)], Y[sub_elem(m, lane & 15)], acc);
   }
   return acc;
 }
@@ -111,7 +150,8 @@ __device__ __forceinline__ f64x4 mm16(const LdsMat &X, int rx, int cx, const Lds
 // Inverse of diagonal block `blk` of op(A) (as the problem is solved: Left
 // form), f64, column-major 64 x 64 at inv + d.inv0 + blk * 4096, zero outside
 // the block's nb x nb corner. An upper block is inverted as the lower one
-// P U P (P reverses the nb indices).
+// P U P (P reverses the nb indices). The padded block is L below nb and the
+// identity past it.
 //
 // When every entry of the block's triangle is finite, blockwise with MFMA:
 // the four 16 x 16 diagonal inverses D_b = L_bb^-1 by substitution (wave b;
@@ -124,23 +164,29 @@ __device__ __forceinline__ f64x4 mm16(const LdsMat &X, int rx, int cx, const Lds
 //
 // Otherwise (an Inf or NaN in the triangle) the whole block by
 // column-oriented substitution against the identity, in registers: lane i
-// holds row i, wave w columns 4w..4w+3; step k scales row k by 1 / L(k, k)
+// holds row i, wave w columns 8w..8w+7; step k scales row k by 1 / L(k, k)
 // and subtracts L(i, k) times it from every later row i, so each entry gets
 // exactly the operations of a row-by-row substitution of its column and a
 // non-finite entry reaches no more of the inverse than it does there.
-template <typename T>
+template <int NT, typename T>
 __device__ __forceinline__ void invert_block(const TrsmDesc<T> &d, int blk, double *inv) {
-  __shared__ LdsMat Ls;  // Ls[k][i] = L(i, k), lower form
-  __shared__ LdsMat Iv;  // Iv[j][i] = inverse(i, j), lower form (MFMA path)
+  static_assert(NT % 64 == 0 && kNB % (NT / 64) == 0, "inversion workgroup shape");
+  __shared__ InvLds S;
   const int64_t r0 = (int64_t)blk * kNB;
   const int nb = (int)min((int64_t)kNB, (int64_t)d.q - r0);
   const int t = threadIdx.x;
-  constexpr int kFill = kNB * kNB / kInvThreads;
+  // Staged: the 6 strictly lower sub-blocks of L into S.ls, the 4 diagonal
+  // ones into their Iv slots; kFill elements per thread.
+  constexpr int kFill = (kInvTri * kSub + NT - 1) / NT;
+  static_assert((kInvTri * kSub) % 64 == 0, "staging rounds are whole waves");
   T raw[kFill];
 #pragma unroll
   for (int u = 0; u < kFill; ++u) {  // all loads issued first, no branches
-    const int idx = t + u * kInvThreads;
-    const int i = idx % kNB, k = idx / kNB;
+    const int idx = min(t + u * NT, kInvTri * kSub - 1);
+    const int q = idx / kSub, e = idx % kSub;
+    const int I = q < kInvOff ? (q < 1 ? 1 : q < 3 ? 2 : 3) : q - kInvOff;
+    const int J = q < kInvOff ? q - I * (I - 1) / 2 : I;
+    const int i = 16 * I + e % 16, k = 16 * J + e / 16;
     const bool in = i < nb && k < nb && i > k;
     const int64_t r = r0 + (d.upper ? nb - 1 - i : i), c = r0 + (d.upper ? nb - 1 - k : k);
     const bool diag = i == k && i < nb && !d.unit;
@@ -150,27 +196,51 @@ __device__ __forceinline__ void invert_block(const TrsmDesc<T> &d, int blk, doub
   bool fin = true;
 #pragma unroll
   for (int u = 0; u < kFill; ++u) {
-    const int idx = t + u * kInvThreads;
-    const int i = idx % kNB, k = idx / kNB;
+    const int idx = t + u * NT;
+    if (idx >= kInvTri * kSub) break;  // wave-uniform: whole waves past the end
+    const int q = idx / kSub, e = idx % kSub;
+    const int I = q < kInvOff ? (q < 1 ? 1 : q < 3 ? 2 : 3) : q - kInvOff;
+    const int J = q < kInvOff ? q - I * (I - 1) / 2 : I;
+    const int i = 16 * I + e % 16, k = 16 * J + e / 16;
     const bool in = i < nb && k < nb && i > k;
     const bool diag = i == k && i < nb && !d.unit;
     const double v = (in || diag) ? (double)raw[u] : (i == k ? 1.0 : 0.0);  // padding: identity
     fin = fin && __builtin_isfinite(v);
-    Ls[k][i] = v;
+    const int o = sub_elem(e % 16, e / 16);
+    if (q < kInvOff)
+      S.ls[q * kSub + o] = v;
+    else
+      S.iv[ld_slot(I) * kSub + o] = v;
   }
-  const bool all_fin = __syncthreads_and(fin);  // also publishes Ls
+  // The barrier below also publishes the staged L. For the batched launch
+  // each wave's finiteness vote goes to Iv slot 8 (the block (3, 2) of the
+  // inverse, written only after a later barrier), so it needs no LDS beyond
+  // InvLds (four per CU); a single call's 16-wave workgroups (two per CU
+  // whatever their LDS) take the barrier's own reduction, one LDS round
+  // trip shorter.
   const int lane = t & 63, w = uni(t >> 6);
+  bool all_fin = true;
+  if constexpr (NT == kInvThreadsSingle) {
+    all_fin = __syncthreads_and(fin);
+  } else {
+    const bool wave_fin = __all(fin);
+    if (lane == 0) S.iv[iv_slot(3, 2) * kSub + w] = wave_fin ? 1.0 : 0.0;
+    __syncthreads();
+#pragma unroll
+    for (int v = 0; v < NT / 64; ++v) all_fin = all_fin && S.iv[iv_slot(3, 2) * kSub + v] != 0.0;
+  }
   double *out = inv + d.inv0 + (int64_t)blk * kNB * kNB;
   if (all_fin) {
-    if (w < 4) {  // D_w by substitution
+    if (w < 4) {  // D_w by substitution, from the L diagonal slot to Iv's diagonal slot
       const int i = lane & 15, g = lane >> 4, o = 16 * w;
-      const double rl = 1.0 / Ls[o + i][o + i];
+      const double *DL = S.iv + ld_slot(w) * kSub;  // L's diagonal sub-block w
+      const double rl = 1.0 / DL[sub_elem(i, i)];
       double x[4];
 #pragma unroll
       for (int r = 0; r < 4; ++r) x[r] = i == g + 4 * r ? 1.0 : 0.0;
 #pragma unroll
       for (int k = 0; k < 16; ++k) {
-        const double lik = Ls[o + k][o + i];
+        const double lik = DL[sub_elem(i, k)];
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
           const double y = x[r] * rl;
@@ -180,46 +250,52 @@ __device__ __forceinline__ void invert_block(const TrsmDesc<T> &d, int blk, doub
         }
       }
 #pragma unroll
-      for (int r = 0; r < 4; ++r) Iv[o + g + 4 * r][o + i] = x[r];
+      for (int r = 0; r < 4; ++r) S.inv(o + i, o + g + 4 * r) = x[r];
     }
+    // Every wave is done with the L diagonal slots before the products below
+    // overwrite them.
     __syncthreads();
 #pragma unroll
     for (int dist = 1; dist < 4; ++dist) {
       if (w < 4 - dist) {  // block (i, j) = (w + dist, w)
         const int bi = w + dist, bj = w;
         f64x4 tacc = {0, 0, 0, 0};
-        for (int bk = bj; bk < bi; ++bk) tacc = mm16(Ls, 16 * bi, 16 * bk, Iv, 16 * bk, 16 * bj, tacc, lane);
+        for (int bk = bj; bk < bi; ++bk) tacc = mm16(S, bi, bk, bj, tacc, lane);
         f64x4 pacc = {0, 0, 0, 0};
+        const double *D = S.iv + iv_slot(bi, bi) * kSub;
 #pragma unroll
         for (int t4 = 0; t4 < 4; ++t4) {
           const int m = 4 * t4 + (lane >> 4);
-          pacc = mma64(Iv[16 * bi + m][16 * bi + (lane & 15)], tacc[t4], pacc);
+          pacc = mma64(D[sub_elem(lane & 15, m)], tacc[t4], pacc);
         }
 #pragma unroll
-        for (int r = 0; r < 4; ++r) Iv[16 * bj + (lane & 15)][16 * bi + (lane >> 4) + 4 * r] = -pacc[r];
+        for (int r = 0; r < 4; ++r)
+          S.inv(16 * bi + (lane >> 4) + 4 * r, 16 * bj + (lane & 15)) = -pacc[r];
       }
       __syncthreads();
     }
+    constexpr int kOut = kNB * kNB / NT;
 #pragma unroll
-    for (int u = 0; u < kFill; ++u) {
-      const int idx = t + u * kInvThreads;
+    for (int u = 0; u < kOut; ++u) {
+      const int idx = t + u * NT;
       const int li = idx % kNB, lj = idx / kNB;
       const bool in = li < nb && lj < nb;
       const int i = (d.upper && in) ? nb - 1 - li : li;
       const int j = (d.upper && in) ? nb - 1 - lj : lj;
-      out[i + j * kNB] = (in && li / 16 >= lj / 16) ? Iv[lj][li] : 0.0;
+      out[i + j * kNB] = (in && li / 16 >= lj / 16) ? S.inv(li, lj) : 0.0;
     }
     return;
   }
-  const int j0 = 4 * w;
-  const double rl = 1.0 / Ls[lane][lane];
-  double x[4];
+  constexpr int kCols = kNB / (NT / 64);  // columns per wave
+  const int j0 = kCols * w;
+  const double rl = 1.0 / S.l(lane, lane);
+  double x[kCols];
 #pragma unroll
-  for (int c = 0; c < 4; ++c) x[c] = lane == j0 + c ? 1.0 : 0.0;
+  for (int c = 0; c < kCols; ++c) x[c] = lane == j0 + c ? 1.0 : 0.0;
   for (int k = j0; k < kNB; ++k) {
-    const double lik = Ls[k][lane];
+    const double lik = S.l(lane, k);
 #pragma unroll
-    for (int c = 0; c < 4; ++c) {
+    for (int c = 0; c < kCols; ++c) {
       if (k >= j0 + c) {  // wave-uniform
         // Every lane forms x / L(lane, lane); lane k's is x_k / L(k, k), the
         // value broadcast (the same product the row-wise substitution forms).
@@ -231,7 +307,7 @@ __device__ __forceinline__ void invert_block(const TrsmDesc<T> &d, int blk, doub
     }
   }
 #pragma unroll
-  for (int c = 0; c < 4; ++c) {
+  for (int c = 0; c < kCols; ++c) {
     const int jl = j0 + c;
     const bool in = lane < nb && jl < nb;
     const int i = (d.upper && lane < nb) ? nb - 1 - lane : lane;
@@ -244,17 +320,17 @@ template <typename T>
 __global__ __launch_bounds__(kInvThreads) void grouped_trsm_inv(const TrsmDesc<T> *descs,
                                                                 const int2 *blocks, double *inv) {
   const int2 u = blocks[blockIdx.x];
-  invert_block(descs[u.x], u.y, inv);
+  invert_block<kInvThreads>(descs[u.x], u.y, inv);
 }
 
 // One problem passed by value: block blockIdx.x. Workgroup 0 also clears
 // the nflags flags of a split solve.
 template <typename T>
-__global__ __launch_bounds__(kInvThreads) void single_trsm_inv(TrsmDesc<T> d, double *inv,
+__global__ __launch_bounds__(kInvThreadsSingle) void single_trsm_inv(TrsmDesc<T> d, double *inv,
                                                                int *flags, int nflags) {
   if (blockIdx.x == 0)
-    for (int i = threadIdx.x; i < nflags; i += kInvThreads) flags[i] = 0;
-  invert_block(d, blockIdx.x, inv);
+    for (int i = threadIdx.x; i < nflags; i += kInvThreadsSingle) flags[i] = 0;
+  invert_block<kInvThreadsSingle>(d, blockIdx.x, inv);
 }
 
 // Address of right-hand side c, row r of the unit's B (Left: B(r, c0 + c);
@@ -1153,7 +1229,7 @@ hipError_t launch_trsm_single(const TrsmDesc<T> &d, double *inv, hipStream_t st)
     xst = inv + (int64_t)nblk * kNB * kNB;
     flags = (int *)(xst + trsm_split_x_doubles(d.q, d.nrhs));
   }
-  hipLaunchKernelGGL((single_trsm_inv<T>), dim3((unsigned)nblk), dim3(kInvThreads), 0, st, d, inv,
+  hipLaunchKernelGGL((single_trsm_inv<T>), dim3((unsigned)nblk), dim3(kInvThreadsSingle), 0, st, d, inv,
                      flags, parts > 1 ? units : 0);
   hipLaunchKernelGGL((single_trsm_solve<T>), dim3((unsigned)(parts * units)), dim3(kThreads), 0,
                      st, d, (const double *)inv, units, parts, flags, xst);
@@ -1164,7 +1240,7 @@ template <typename T>
 hipError_t launch_trsm_invert(const TrsmDesc<T> &d, double *inv, hipStream_t st) {
   if (d.q <= 0) return hipSuccess;
   const int nblk = (d.q + kNB - 1) / kNB;
-  hipLaunchKernelGGL((single_trsm_inv<T>), dim3((unsigned)nblk), dim3(kInvThreads), 0, st, d, inv,
+  hipLaunchKernelGGL((single_trsm_inv<T>), dim3((unsigned)nblk), dim3(kInvThreadsSingle), 0, st, d, inv,
                      (int *)nullptr, 0);
   return hipGetLastError();
 }
