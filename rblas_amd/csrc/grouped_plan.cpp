@@ -29,6 +29,32 @@ struct Shape {
   double unit_us;
 };
 
+// k-loop time of a tile reaching rem_m x rem_n into the matrix, as a share of
+// a full tile's: wave w holds row blocks 2i + w / 4 and column blocks
+// 4j + w % 4 and skips those past the edge, so the tile takes about as long
+// as its busiest SIMD's share of the blocks allows, above a floor (fitted
+// to edge tiles timed alone on MI355X: 128-tiles 0.215 + 0.76 share,
+// 64-tiles 0.46 + 0.52 share).
+double edge_scale(int64_t rem_m, int64_t rem_n, int tile) {
+  const int fm = tile / 32, fn = tile / 64;
+  const int rb = (int)((std::min<int64_t>(rem_m, tile) + 15) / 16);
+  const int cb = (int)((std::min<int64_t>(rem_n, tile) + 15) / 16);
+  const int nbi = std::max(0, std::min(fm, (rb + 1) >> 1)) + std::max(0, std::min(fm, rb >> 1));
+  int busiest = 0;
+  for (int simd = 0; simd < 4; ++simd)
+    busiest = std::max(busiest, std::max(0, std::min(fn, (cb - simd + 3) >> 2)) * nbi);
+  if (busiest >= 2 * fm * fn) return 1.0;
+  const double share = (double)busiest / (2 * fm * fn);
+  return tile == 128 ? 0.215 + 0.76 * share : 0.46 + 0.52 * share;
+}
+
+// Sum of the k-loop scales of a tm x tn grid of tiles over m x n.
+double grid_scale(int64_t m, int64_t n, int tile, int64_t tm, int64_t tn) {
+  const int64_t em = m - (tm - 1) * tile, en = n - (tn - 1) * tile;  // edge extents
+  return (double)((tm - 1) * (tn - 1)) + (double)(tn - 1) * edge_scale(em, tile, tile) +
+         (double)(tm - 1) * edge_scale(tile, en, tile) + edge_scale(em, en, tile);
+}
+
 // The launch shape of one product problem: a function of (dtype, m, n, k)
 // and the tuning table only, so a problem gives the same bits in any group.
 Shape plan_shape(const Tuning &tu, int dtype, int64_t m, int64_t n, int64_t k) {
@@ -39,7 +65,8 @@ Shape plan_shape(const Tuning &tu, int dtype, int64_t m, int64_t n, int64_t k) {
     const int tile = ti ? 128 : 64;
     const int64_t tm = (m + tile - 1) / tile, tn = (n + tile - 1) / tile;
     const double unit = tu.c0[dtype][ti] + tu.c1[dtype][ti] * (double)kt;
-    const double total = (double)(tm * tn) * unit;
+    const double total = (double)(tm * tn) * tu.c0[dtype][ti] +
+                         tu.c1[dtype][ti] * (double)kt * grid_scale(m, n, tile, tm, tn);
     if (ti == 1 || total < best_total) {
       best = Shape{tile, 1, (int)tm, (int)(tm * tn), unit};
       best_total = total;
@@ -140,12 +167,19 @@ void GemmGroup<T>::add(const GemmProblem<T> &p, const Tuning &tu) {
     ++info.split_problems;
   }
   (sh.tile == 128 ? info.tiles128 : info.tiles64) += sh.tiles;
-  for (int t = 0; t < sh.tiles; ++t)
+  const int ti = sh.tile == 128 ? 1 : 0;
+  const double fixed = tu.c0[dtype][ti], kloop = sh.unit_us - fixed;
+  for (int t = 0; t < sh.tiles; ++t) {
+    // An edge tile's units are priced by the share of the k loop it runs.
+    const int64_t rem_m = p.m - (int64_t)(t % sh.tiles_m) * sh.tile;
+    const int64_t rem_n = p.n - (int64_t)(t / sh.tiles_m) * sh.tile;
+    const double u = fixed + kloop * edge_scale(rem_m, rem_n, sh.tile);
     for (int s = 0; s < sh.splits; ++s) {
       units.push_back(WorkUnit{prob, t, s, 0});
-      cost.push_back(sh.unit_us);
-      total_us += sh.unit_us;
+      cost.push_back(u);
+      total_us += u;
     }
+  }
   info.flops += 2.0 * (double)p.m * (double)p.n * (double)p.k;
   descs.push_back(d);
 }
