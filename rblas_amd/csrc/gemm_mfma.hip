@@ -121,6 +121,7 @@ template <typename T>
 GemmConfig gemm_config(const GemmProblem<T> &p) {
   GemmConfig cfg{128, 1};
   if (p.tri != 0) return cfg;
+  if (gemm_wide_ok(p)) return GemmConfig{256, 1};
   const int64_t t128 = ((p.m + 127) / 128) * ((p.n + 127) / 128);
   const int64_t t64 = ((p.m + 63) / 64) * ((p.n + 63) / 64);
   const int64_t ktiles = (p.k + Geo<T>::BK - 1) / Geo<T>::BK;
@@ -154,6 +155,7 @@ size_t gemm_workspace_bytes(const GemmProblem<T> &p, GemmConfig cfg) {
 
 template <typename T>
 hipError_t launch_gemm_cfg(const GemmProblem<T> &p, GemmConfig cfg, void *ws, hipStream_t st) {
+  if (cfg.tile == 256) return cfg.splits == 1 ? launch_gemm_wide(p, st) : hipErrorInvalidValue;
   if (!gemm_fits(p)) return hipErrorInvalidValue;
   if (p.tri != 0 && (p.m != p.n || cfg.splits != 1)) return hipErrorInvalidValue;
   if (cfg.tile != 64 && cfg.tile != 128) return hipErrorInvalidValue;

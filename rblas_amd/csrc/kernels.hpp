@@ -30,9 +30,11 @@ struct GemmProblem {
 // Compute units of one MI355X (8 XCDs x 32).
 constexpr int kCUs = 256;
 
-// Launch shape of one MFMA gemm: C tile side (64 or 128) and the number of
-// k splits (1 = none). A pure function of the problem's shape, so a problem
-// gives the same bits whenever and with whatever it runs.
+// Launch shape of one MFMA gemm: C tile side (64 or 128; 256 = the 128 x 256
+// wide tile of gemm_wide.hip) and the number of k splits (1 = none). A pure
+// function of the problem's shape; the k order of every C entry is the same
+// for every tile, so a problem gives the same bits whenever and with
+// whatever it runs.
 struct GemmConfig {
   int tile;
   int splits;
@@ -56,6 +58,13 @@ hipError_t launch_gemm_cfg(const GemmProblem<T> &p, GemmConfig cfg, void *ws,
 // Whether every in-tile operand offset fits the 31-bit buffer addressing.
 template <typename T>
 bool gemm_fits(const GemmProblem<T> &p);
+// The 128 x 256 tile (gemm_wide.hip): whether it applies (op(B) stored
+// K-contiguous, no triangle, at least two tiles per CU, its in-tile offsets
+// within 31 bits), and its launch.
+template <typename T>
+bool gemm_wide_ok(const GemmProblem<T> &p);
+template <typename T>
+hipError_t launch_gemm_wide(const GemmProblem<T> &p, hipStream_t stream);
 
 // C := beta * C over an m x n block (rblas apply_beta, kernel/gemm.rs:322-338):
 // beta == 0 stores exact zeros, beta == 1 is a no-op the host skips.

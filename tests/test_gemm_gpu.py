@@ -305,3 +305,40 @@ def test_try_gemm_ok_equals_gemm_bitwise(gpu, dtype):
         assert np.array_equal(c_try, c_gemm, equal_nan=True)
         checked += 1
     assert checked == 9 * 3 * 2 * 3
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("layout,ta,tb", [(COL, 0, 0), (COL, 1, 0), (ROW, 0, 0), (ROW, 0, 1)])
+def test_wide_tile_edges_match_grouped(gpu, dtype, layout, ta, tb):
+    """Shapes large enough for the 128 x 256 tile (op(B) stored K-contiguous
+    after the RowMajor swap) with ragged edges in m, n and k: that tile
+    computes every block of an edge tile instead of skipping those past the
+    edge, and the k order of each C entry is the 128 x 128 / 64 x 64 tiles'
+    one, so the single call agrees bit for bit with a grouped plan of the
+    same problem (which runs 128 / 64 tiles); both meet the float64 bound on
+    sampled rows. (Row-major NN and NT map to col-major NN and TN.)"""
+    from rblas_amd import grouped
+    from tests.util import op
+    m, n, k = 4000 + 3, 4100 + 29, 333
+    a, lda, b, ldb, c, ldc = make_problem(layout, ta, tb, m, n, k, 5, dtype, 9)
+    c0 = c.copy()
+    s = Stream()
+    da, db, dc, dg = (DeviceArray.from_host(x, s) for x in (a, b, c, c))
+    level3.gemm_dev(layout, ta, tb, m, n, k, 1.5, da, lda, db, ldb, -0.25, dc, ldc, s)
+    plan = grouped.GroupedGemm([grouped.GemmProblem(layout, ta, tb, m, n, k, 1.5, da, lda, db,
+                                                    ldb, -0.25, dg, ldc)], dtype)
+    plan.run(s)
+    s.synchronize()
+    single, group = dc.download(s), dg.download(s)
+    plan.close()
+    s.close()
+    assert np.array_equal(single, group)
+    got = view(single, layout, m, n, ldc)
+    A = op(a, layout, ta, m, k, lda).astype(np.float64)
+    B = op(b, layout, tb, k, n, ldb).astype(np.float64)
+    C0 = view(c0, layout, m, n, ldc).astype(np.float64)
+    rows = np.concatenate([np.random.default_rng(1).choice(m, 40, replace=False),
+                           np.arange(m - 8, m)])  # the last row tile too
+    want = 1.5 * (A[rows] @ B) - 0.25 * C0[rows]
+    mag = 1.5 * (np.abs(A[rows]) @ np.abs(B)) + 0.25 * np.abs(C0[rows])
+    assert_gemm_close(got[rows], want, mag, k, dtype, what="wide tile")
