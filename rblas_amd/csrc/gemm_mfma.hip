@@ -5,7 +5,7 @@
 // This replaces rblas's Goto/BLIS pipeline (kernel/gemm.rs:185-474 and the
 // AVX2 microkernels gemm_avx2.rs:26-155):
 //   - pack_a_panel / pack_b_panel -> the buffer unit's direct global->LDS
-//     loads (`buffer_load_dwordx4 ... lds`), four LDS stages in flight, one
+//     loads (`buffer_load_dwordx4 ... lds`), three LDS stages in flight, one
 //     barrier per k-tile, no staging registers;
 //   - the MR x NR register microtile -> a per-wave block of 16 x 16 MFMA
 //     accumulators (64 x 32 per wave in a 128 x 128 tile, 8 waves);

@@ -512,7 +512,7 @@ struct TileGemm {
     if constexpr (kStages > 3) issue(2, lds_a<2>(), lds_b<2>());
   }
 
-  // The k loop after the prologue, unrolled by the 4 stages, in a copy for
+  // The k loop after the prologue, unrolled by the kStages stages, in a copy for
   // each count of this wave's row and column blocks that reach into the
   // matrix (wave-uniform): all of them on an interior tile, fewer on an edge
   // tile, none for a wave wholly past the edge (it only stages and waits).
