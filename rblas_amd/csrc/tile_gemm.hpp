@@ -457,10 +457,11 @@ struct TileGemm {
   }
 
   // Pipeline step for k-tile kt, which lives in stage S. f32 issues the DMA
-  // of tile kt+3 after its first fragment group's MFMAs, so the issue's
-  // address work overlaps MFMAs already in flight (+1.4% mean over the eight
-  // SGEMM 8192 layouts); f64 issues it right after the barrier, which
-  // measured faster there. The k order, and so every result bit, is the same.
+  // of tile kt + kStages - 1 after its first fragment group's MFMAs, so the
+  // issue's address work overlaps MFMAs already in flight (+1.4% mean over
+  // the eight SGEMM 8192 layouts, measured with four stages); f64 issues it
+  // right after the barrier, which measured faster there. The k order, and
+  // so every result bit, is the same.
   template <int S, int NI, int NJ>
   __device__ __forceinline__ void step(int kt) {
     if (kt >= ktiles) return;
