@@ -9,12 +9,11 @@
 // the 128 x 128 and 64 x 64 tiles (tile_gemm.hpp), so the results are the
 // same bits.
 //
-// Used for op(B) stored K-contiguous (B not transposed, after the RowMajor
-// swap): a 256-wide M/N-contiguous f64 operand would need two LDS blocks per
-// k-row, which the staging layout does not have. Measured at 8192^3 (bench,
-// HBM-resident operands): DGEMM NN 65.9 -> 68.4 TFLOP/s, TN 65.6 -> 67.2;
-// SGEMM NN 126-128 -> 131.8; ZGEMM 4096^3 (4M, its real products here)
-// 62.8 -> 66.2.
+// Used (after the RowMajor swap) for f64 with op(B) stored K-contiguous (NN,
+// TN) and for f32 with op(A) stored M-contiguous (NN, NT); see
+// gemm_wide_shape_ok. Measured at 8192^3 (bench, HBM-resident operands):
+// DGEMM NN 65.9 -> 68.4 TFLOP/s, TN 65.6 -> 67.1; SGEMM NN 128 -> 131.8,
+// NT 129.5 -> 133.9; ZGEMM 4096^3 (4M, its real products here) 62.8 -> 66.2.
 #define RBLAS_TILE_STAGES 3
 #define RBLAS_TILE_MAX_B 256
 #include "tile_gemm.hpp"
@@ -24,9 +23,9 @@ namespace {
 
 constexpr int kWideM = 128, kWideN = 256;
 
-template <typename T, bool TA>
+template <typename T, bool TA, bool TB>
 __global__ __launch_bounds__(kThreads) void gemm_wide_kernel(GemmProblem<T> p) {
-  TileGemm<T, TA, false, kWideM, kWideN, false> tg;
+  TileGemm<T, TA, TB, kWideM, kWideN, false> tg;
   const int tiles_m = (int)((p.m + kWideM - 1) / kWideM);
   const int tiles_n = (int)((p.n + kWideN - 1) / kWideN);
   int tm, tn;
@@ -41,11 +40,13 @@ __global__ __launch_bounds__(kThreads) void gemm_wide_kernel(GemmProblem<T> p) {
 
 template <typename T>
 bool gemm_wide_ok(const GemmProblem<T> &p) {
-  if (p.tri != 0 || p.tb) return false;
-  // f32 with op(A) stored K-contiguous measured slower wide (SGEMM 8192^3
-  // TN 125.6 against 127.9 TFLOP/s with 128 x 128 tiles; f64 TN 67.2
-  // against 65.6).
-  if (sizeof(T) == 4 && p.ta) return false;
+  if (p.tri != 0) return false;
+  // f64: op(B) K-contiguous (a 256-wide M/N-contiguous f64 operand needs two
+  // LDS blocks per k-row). f32: op(A) M-contiguous -- with op(A) stored
+  // K-contiguous the wide tile measured slower (SGEMM 8192^3 TN 125.6
+  // against 130.2 TFLOP/s with 128 x 128 tiles), while NT gains (129.5 ->
+  // 133.9; a 256-wide f32 k-row is exactly one 1 KiB LDS block).
+  if (sizeof(T) == 8 ? p.tb : p.ta) return false;
   const int64_t tiles = ((p.m + kWideM - 1) / kWideM) * ((p.n + kWideN - 1) / kWideN);
   if (tiles < 2 * kCUs) return false;  // fills the CUs at least twice
   // Largest in-tile byte offsets (31-bit buffer addressing): op(A) over 128
@@ -63,11 +64,14 @@ hipError_t launch_gemm_wide(const GemmProblem<T> &p, hipStream_t st) {
   if (!gemm_wide_ok(p)) return hipErrorInvalidValue;
   const int64_t tiles = ((p.m + kWideM - 1) / kWideM) * ((p.n + kWideN - 1) / kWideN);
   if (p.ta)
-    hipLaunchKernelGGL((gemm_wide_kernel<T, true>), dim3((unsigned)tiles), dim3(kThreads), 0, st,
-                       p);
+    hipLaunchKernelGGL((gemm_wide_kernel<T, true, false>), dim3((unsigned)tiles), dim3(kThreads),
+                       0, st, p);
+  else if (!p.tb)
+    hipLaunchKernelGGL((gemm_wide_kernel<T, false, false>), dim3((unsigned)tiles), dim3(kThreads),
+                       0, st, p);
   else
-    hipLaunchKernelGGL((gemm_wide_kernel<T, false>), dim3((unsigned)tiles), dim3(kThreads), 0, st,
-                       p);
+    hipLaunchKernelGGL((gemm_wide_kernel<T, false, true>), dim3((unsigned)tiles), dim3(kThreads),
+                       0, st, p);
   return hipGetLastError();
 }
 
