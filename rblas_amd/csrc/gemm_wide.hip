@@ -2,18 +2,18 @@
 // 64 x 64 (four by four 16 x 16 f64 / f32 MFMA blocks each), so every LDS
 // barrier and every fragment read feeds twice the MFMA work of the 128 x 128
 // tile. The stages are larger (A 128 x BK, B 256 x BK), so this translation
-// unit keeps three of them (52 KiB each, 156 KiB of the 160 KiB LDS), and
+// unit keeps three of them (53 KiB each, 159 KiB of the 160 KiB LDS), and
 // its waves compute every block of an edge tile rather than skipping those
 // past the edge (the k-loop copies that skipping needs cost more registers
 // than the tile has to spare). The k order of every C entry is the one of
 // the 128 x 128 and 64 x 64 tiles (tile_gemm.hpp), so the results are the
 // same bits.
 //
-// Used (after the RowMajor swap) for f64 with op(B) stored K-contiguous (NN,
-// TN) and for f32 with op(A) stored M-contiguous (NN, NT); see
-// gemm_wide_shape_ok. Measured at 8192^3 (bench, HBM-resident operands):
-// DGEMM NN 65.9 -> 68.4 TFLOP/s, TN 65.6 -> 67.1; SGEMM NN 128 -> 131.8,
-// NT 129.5 -> 133.9; ZGEMM 4096^3 (4M, its real products here) 62.8 -> 66.2.
+// Used (after the RowMajor swap) for f64 NN, TN and NT and for f32 NN and NT;
+// see gemm_wide_shape_ok. Measured at 8192^3 (bench, HBM-resident operands):
+// DGEMM NN 65.9 -> 68.4 TFLOP/s, TN 65.6 -> 67.1, NT 65.7 -> 67.6; SGEMM NN
+// 128 -> 131.8, NT 129.5 -> 134.9; ZGEMM 4096^3 (4M, its real products
+// here) 62.8 -> 66.2.
 #define RBLAS_TILE_STAGES 3
 #define RBLAS_TILE_MAX_B 256
 #include "tile_gemm.hpp"
@@ -41,12 +41,12 @@ __global__ __launch_bounds__(kThreads) void gemm_wide_kernel(GemmProblem<T> p) {
 template <typename T>
 bool gemm_wide_ok(const GemmProblem<T> &p) {
   if (p.tri != 0) return false;
-  // f64: op(B) K-contiguous (a 256-wide M/N-contiguous f64 operand needs two
-  // LDS blocks per k-row). f32: op(A) M-contiguous -- with op(A) stored
-  // K-contiguous the wide tile measured slower (SGEMM 8192^3 TN 125.6
-  // against 130.2 TFLOP/s with 128 x 128 tiles), while NT gains (129.5 ->
-  // 133.9; a 256-wide f32 k-row is exactly one 1 KiB LDS block).
-  if (sizeof(T) == 8 ? p.tb : p.ta) return false;
+  // The layouts where it measured faster at 8192^3 (TFLOP/s, against 128 x
+  // 128 tiles): f64 NN 68.4 / 65.9, TN 67.1 / 65.6, NT 67.6 / 65.7 (TT 62.6
+  // / 65.7: its kernel spills); f32 NN 131.9 / 128, NT 134.9 / 129.5 (TN
+  // 125.6 / 130.2). So op(A) stored K-contiguous takes it only in f64 with
+  // op(B) stored K-contiguous too.
+  if (p.ta && (sizeof(T) == 4 || p.tb)) return false;
   const int64_t tiles = ((p.m + kWideM - 1) / kWideM) * ((p.n + kWideN - 1) / kWideN);
   if (tiles < 2 * kCUs) return false;  // fills the CUs at least twice
   // Largest in-tile byte offsets (31-bit buffer addressing): op(A) over 128
@@ -63,15 +63,15 @@ template <typename T>
 hipError_t launch_gemm_wide(const GemmProblem<T> &p, hipStream_t st) {
   if (!gemm_wide_ok(p)) return hipErrorInvalidValue;
   const int64_t tiles = ((p.m + kWideM - 1) / kWideM) * ((p.n + kWideN - 1) / kWideN);
-  if (p.ta)
-    hipLaunchKernelGGL((gemm_wide_kernel<T, true, false>), dim3((unsigned)tiles), dim3(kThreads),
-                       0, st, p);
+  const dim3 grid((unsigned)tiles), block(kThreads);
+  if (!p.ta && !p.tb)
+    hipLaunchKernelGGL((gemm_wide_kernel<T, false, false>), grid, block, 0, st, p);
+  else if (!p.ta)
+    hipLaunchKernelGGL((gemm_wide_kernel<T, false, true>), grid, block, 0, st, p);
   else if (!p.tb)
-    hipLaunchKernelGGL((gemm_wide_kernel<T, false, false>), dim3((unsigned)tiles), dim3(kThreads),
-                       0, st, p);
+    hipLaunchKernelGGL((gemm_wide_kernel<T, true, false>), grid, block, 0, st, p);
   else
-    hipLaunchKernelGGL((gemm_wide_kernel<T, false, true>), dim3((unsigned)tiles), dim3(kThreads),
-                       0, st, p);
+    hipLaunchKernelGGL((gemm_wide_kernel<T, true, true>), grid, block, 0, st, p);
   return hipGetLastError();
 }
 

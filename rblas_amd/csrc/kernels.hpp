@@ -58,9 +58,9 @@ hipError_t launch_gemm_cfg(const GemmProblem<T> &p, GemmConfig cfg, void *ws,
 // Whether every in-tile operand offset fits the 31-bit buffer addressing.
 template <typename T>
 bool gemm_fits(const GemmProblem<T> &p);
-// The 128 x 256 tile (gemm_wide.hip): whether it applies (f64: op(B) stored
-// K-contiguous; f32: op(A) stored M-contiguous; no triangle, at least two
-// tiles per CU, its in-tile offsets within 31 bits), and its launch.
+// The 128 x 256 tile (gemm_wide.hip): whether it applies (f64 NN, TN, NT;
+// f32 NN, NT; no triangle, at least two tiles per CU, its in-tile offsets
+// within 31 bits), and its launch.
 template <typename T>
 bool gemm_wide_ok(const GemmProblem<T> &p);
 template <typename T>

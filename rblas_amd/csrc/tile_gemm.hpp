@@ -20,7 +20,10 @@
 // fall on different banks:
 //   M/N-contiguous operand: block q holds tile k-rows q*R .. q*R+R-1
 //     (R = 1024 / (TILE * sizeof(T))) as [row][TILE], lane l carrying V
-//     consecutive m (or n) of one row; block stride 1024 + 128/V bytes.
+//     consecutive m (or n) of one row; block stride 1024 + 128/V bytes. A
+//     k-row longer than a block (f64, TILE = 256) spans BPR = TILE * 8 /
+//     1024 consecutive blocks, stride 1024 + 128 bytes (the lane groups of a
+//     fragment read, 2 k apart, then land 32 banks apart).
 //   K-contiguous operand: block q holds the 8 m (or n) q*8 .. q*8+7, each as
 //     its BK k (128 bytes = eight 16-byte chunks of V k). Lanes 8r .. 8r+7
 //     fetch the eight chunks of one column, so each instruction reads whole
@@ -91,7 +94,8 @@ constexpr int kMaxTile = 128;  // largest C tile side along m (and n unless RBLA
 constexpr int kStages = RBLAS_TILE_STAGES;  // LDS pipeline depth
 static_assert(kStages == 3 || kStages == 4, "LDS pipeline depth");
 constexpr int kOperandBytes = (kMaxTile / 8) * (1024 + 64);  // largest operand stage
-constexpr int kOperandBytesB = (RBLAS_TILE_MAX_B / 8) * (1024 + 64);
+constexpr int kOperandBytesB =
+    (RBLAS_TILE_MAX_B / 8) * (RBLAS_TILE_MAX_B > 128 ? 1024 + 128 : 1024 + 64);
 constexpr uint32_t kMasked = 0x80000000u;  // buffer offset past num_records: reads 0
 
 template <typename T>
@@ -102,9 +106,10 @@ struct Geo {
 };
 
 // Byte stride between DMA blocks of one operand tile.
-template <typename T, bool MN_MAJOR>
+template <typename T, bool MN_MAJOR, int TILE>
 constexpr int block_bytes() {
-  return MN_MAJOR ? 1024 + 128 / Geo<T>::V : 1024 + 16;
+  if (!MN_MAJOR) return 1024 + 16;
+  return TILE * (int)sizeof(T) > 1024 ? 1024 + 128 : 1024 + 128 / Geo<T>::V;
 }
 
 // LDS: kStages stages of (A tile, B tile), one distinct object per stage so
@@ -178,9 +183,16 @@ __device__ __forceinline__ void buffer_store<float>(__amdgpu_buffer_rsrc_t r, ui
 template <typename T, bool MN_MAJOR, int TILE>
 __device__ __forceinline__ int lds_byte(int k, int mn) {
   typedef Geo<T> G;
-  constexpr int bb = block_bytes<T, MN_MAJOR>();
-  constexpr int R = 1024 / (TILE * (int)sizeof(T));
-  if (MN_MAJOR) return (k / R) * bb + ((k % R) * TILE + mn) * (int)sizeof(T);
+  constexpr int bb = block_bytes<T, MN_MAJOR, TILE>();
+  if constexpr (MN_MAJOR) {
+    if constexpr (TILE * (int)sizeof(T) <= 1024) {
+      constexpr int R = 1024 / (TILE * (int)sizeof(T));
+      return (k / R) * bb + ((k % R) * TILE + mn) * (int)sizeof(T);
+    } else {
+      constexpr int per = 1024 / (int)sizeof(T), bpr = TILE / per;  // elements, blocks a k-row
+      return (k * bpr + mn / per) * bb + (mn % per) * (int)sizeof(T);
+    }
+  }
   const int r = mn % 8;
   return (mn / 8) * bb + r * 128 + (((k / G::V) ^ r) * 16) + (k % G::V) * (int)sizeof(T);
 }
@@ -190,7 +202,6 @@ template <typename T, bool MN_MAJOR, int TILE>
 struct Operand {
   typedef Geo<T> G;
   static constexpr int NB = TILE / (8 * kWaves);  // DMA blocks per wave per k-tile
-  static constexpr int R = 1024 / (TILE * (int)sizeof(T));
   const T *x;
   const T *end;  // one past the last element the stored operand may touch
   int64_t ld;
@@ -210,10 +221,17 @@ struct Operand {
     for (int j = 0; j < NB; ++j) {
       const int q = block0 + j;
       int k, mn;
-      if (MN_MAJOR) {
+      if constexpr (MN_MAJOR) {
         const int e = lane * G::V;
-        k = q * R + e / TILE;
-        mn = e % TILE;
+        if constexpr (TILE * (int)sizeof(T) <= 1024) {
+          constexpr int R = 1024 / (TILE * (int)sizeof(T));  // k-rows a block
+          k = q * R + e / TILE;
+          mn = e % TILE;
+        } else {
+          constexpr int per = 1024 / (int)sizeof(T), bpr = TILE / per;  // blocks a k-row
+          k = q / bpr;
+          mn = (q % bpr) * per + e;
+        }
       } else {
         const int r = lane / 8;            // column within the block
         k = ((lane % 8) ^ r) * G::V;       // chunk stored in slot lane % 8
@@ -236,8 +254,8 @@ struct Operand {
 #pragma unroll
     for (int j = 0; j < NB; ++j)
       __builtin_amdgcn_raw_ptr_buffer_load_lds(
-          rs, (lds_void_ptr)(lds + (block0 + j) * block_bytes<T, MN_MAJOR>()), 16, off[j], 0, 0,
-          0);
+          rs, (lds_void_ptr)(lds + (block0 + j) * block_bytes<T, MN_MAJOR, TILE>()), 16, off[j],
+          0, 0, 0);
   }
 
   // Partial last k-tile: element-wise masked loads, written in the same

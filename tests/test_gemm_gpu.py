@@ -308,17 +308,16 @@ def test_try_gemm_ok_equals_gemm_bitwise(gpu, dtype):
 
 
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
-@pytest.mark.parametrize("layout,ta,tb", [(COL, 0, 0), (COL, 1, 0), (COL, 0, 1), (ROW, 0, 0),
-                                          (ROW, 0, 1), (ROW, 1, 0)])
+@pytest.mark.parametrize("layout,ta,tb", list(itertools.product([COL, ROW], [0, 1], [0, 1])))
 def test_wide_tile_edges_match_grouped(gpu, dtype, layout, ta, tb):
-    """Shapes large enough for the 128 x 256 tile (f64: op(B) stored
-    K-contiguous, f32: op(A) stored M-contiguous, after the RowMajor swap;
-    the other cases run 128 x 128 tiles) with ragged edges in m, n and k: that tile
+    """Shapes large enough for the 128 x 256 tile (every layout for f64, op(A)
+    stored M-contiguous for f32, after the RowMajor swap; the other cases
+    run 128 x 128 tiles) with ragged edges in m, n and k: that tile
     computes every block of an edge tile instead of skipping those past the
     edge, and the k order of each C entry is the 128 x 128 / 64 x 64 tiles'
     one, so the single call agrees bit for bit with a grouped plan of the
     same problem (which runs 128 / 64 tiles); both meet the float64 bound on
-    sampled rows. (Row-major NN, NT and TN map to col-major NN, TN and NT.)"""
+    sampled rows."""
     from rblas_amd import grouped
     from tests.util import op
     m, n, k = 4000 + 3, 4100 + 29, 333
