@@ -309,7 +309,8 @@ def test_try_gemm_ok_equals_gemm_bitwise(gpu, dtype):
 
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
 @pytest.mark.parametrize("layout,ta,tb", list(itertools.product([COL, ROW], [0, 1], [0, 1])))
-def test_wide_tile_edges_match_grouped(gpu, dtype, layout, ta, tb):
+@pytest.mark.parametrize("k", [333, 5])
+def test_wide_tile_edges_match_grouped(gpu, dtype, layout, ta, tb, k):
     """Shapes large enough for the 128 x 256 tile (every layout for f64, op(A)
     stored M-contiguous for f32, after the RowMajor swap; the other cases
     run 128 x 128 tiles) with ragged edges in m, n and k: that tile
@@ -320,7 +321,7 @@ def test_wide_tile_edges_match_grouped(gpu, dtype, layout, ta, tb):
     sampled rows."""
     from rblas_amd import grouped
     from tests.util import op
-    m, n, k = 4000 + 3, 4100 + 29, 333
+    m, n = 4000 + 3, 4100 + 29  # k = 5: one partial k-tile, staged through registers
     a, lda, b, ldb, c, ldc = make_problem(layout, ta, tb, m, n, k, 5, dtype, 9)
     c0 = c.copy()
     s = Stream()
