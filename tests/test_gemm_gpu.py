@@ -311,9 +311,10 @@ def test_try_gemm_ok_equals_gemm_bitwise(gpu, dtype):
 @pytest.mark.parametrize("layout,ta,tb", list(itertools.product([COL, ROW], [0, 1], [0, 1])))
 @pytest.mark.parametrize("k", [333, 5])
 def test_wide_tile_edges_match_grouped(gpu, dtype, layout, ta, tb, k):
-    """Shapes large enough for the 128 x 256 tile (every layout for f64, op(A)
-    stored M-contiguous for f32, after the RowMajor swap; the other cases
-    run 128 x 128 tiles) with ragged edges in m, n and k: that tile
+    """Shapes large enough for the 128 x 256 tile (after the RowMajor swap:
+    f64 unless op(A) and op(B) are both stored K-contiguous, f32 with op(A)
+    stored M-contiguous; the other cases run 128 x 128 tiles) with ragged
+    edges in m, n and k: that tile
     computes every block of an edge tile instead of skipping those past the
     edge, and the k order of each C entry is the 128 x 128 / 64 x 64 tiles'
     one, so the single call agrees bit for bit with a grouped plan of the
